@@ -1,0 +1,132 @@
+"""Benchmark: amplitude updates per second of the dense complex128 state
+vector on one MI355X, on the 30-qubit rotation brickwork (depth 24: RX/RY/RZ
+on every qubit, uniform axis and angle, then CZ on alternating neighbour
+pairs) followed by 64 random Pauli expectations (weight 1-4).
+
+One step = reset to |0...0>, run the circuit, compute the 64 expectations.
+amp-updates = (unfused gate count) x 2^n per step.  Prints one JSON line with
+the roofline of the dominant kernel (HIP-event timed on the state's stream)
+and a CPU baseline (the C oracle, a bounded sample of the same circuit family
+on this host's cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle (C restatement of StateVector<C64>, OpenMP) applying the
+    26-qubit member of the same circuit family gate by gate until `seconds`
+    have passed."""
+    import numpy as np
+
+    from oracle.circuit import Circuit as OCircuit
+    from oracle.statevector import StateVector, lib
+    from ncsim_amd import workloads as W
+
+    n = 26
+    circ = OCircuit.parse(W.rotation_brickwork(n, 24, 3))
+    v = StateVector.zero_state(n)
+    t0 = time.perf_counter()
+    done = 0
+    for op in circ.ops:
+        v.apply_gate(op[1], op[2])
+        done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    threads = lib().orc_num_threads()
+    del v
+    return {"value": done * float(1 << n) / dt, "unit": "amp-updates/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} gates of the 26-qubit depth-24 rotation brickwork, "
+                      f"{dt:.1f} s, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=24)
+    ap.add_argument("--fusion", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only: the state-vector path runs on a single "
+                                   "MI355X; sharding across GPUs is described in DESIGN.md but not built"}))
+        sys.exit(2)
+
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+
+    n, depth = args.qubits, args.depth
+    text = W.rotation_brickwork(n, depth, 3)
+    gates = W.count_gates(text)
+    paulis = [H.Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 64, 4)]
+    circ = H.Circuit.parse(text)
+    log(f"bench: n={n} depth={depth} gates={gates} paulis=64 fusion={args.fusion}")
+    s = H.HipStateVector.zero_state(n, H.C128)
+    s.set_fusion(args.fusion)
+
+    def step():
+        s.run(circ)
+        return s.expectations(paulis)
+
+    for i in range(args.warmup):
+        H._lib.lib().ncsv_set_basis(s._h, 0)
+        step()
+        log(f"bench: warmup {i} done")
+    s.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        H._lib.lib().ncsv_set_basis(s._h, 0)
+        step()
+        log(f"bench: step {i} done")
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1000.0 / args.steps
+    value = gates * float(1 << n) / (ms / 1000.0)
+
+    # one profiled step: per-kernel durations by HIP events on the state's stream
+    H._lib.lib().ncsv_set_basis(s._h, 0)
+    s.profile_reset()
+    s.profile(True)
+    step()
+    s.profile(False)
+    stats = s.profile_read()
+    sweeps = s.sweep_count()
+    name, (launches, tot_ms, byts) = max(stats.items(), key=lambda kv: kv[1][1])
+    avg_s = tot_ms / launches / 1000.0
+    achieved = byts / launches / avg_s
+    log("bench: kernels " + ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(stats.items())))
+    roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "launches": launches,
+                "avg_ms": tot_ms / launches, "sweeps_per_step": sweeps}
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "amp-updates/s", "value": value, "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (f64)", "data": "synthetic (seeded circuit)",
+        "config": {"workload": f"rotation brickwork {n} qubits depth {depth} + 64 Pauli expectations",
+                   "qubits": n, "depth": depth, "gates": gates, "fusion": args.fusion, "precision": "complex128"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

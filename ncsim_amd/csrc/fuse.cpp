@@ -1,0 +1,642 @@
+// Gate-segment execution and the fusion planner.
+//
+// Fusion mode 0 applies one device sweep per gate, the reference's schedule
+// (vector.rs:364-395 per op).  Mode 1 plans tile sweeps: gates are taken in
+// circuit order as long as they do not depend on a gate left for a later
+// sweep (two gates commute when every qubit they share is one both act on
+// diagonally), and each sweep's tile qubits are chosen greedily to admit the
+// most gates.  Within a sweep the same rule, with R register qubits instead
+// of L tile qubits, splits the gates into stages.
+#include "fuse.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace ncsv {
+
+namespace {
+
+// A lowered gate as a tile op on global qubits, plus its dependency masks.
+struct Cls {
+    bool tileable = false;
+    uint64_t act = 0;  // qubits it acts on non-diagonally (must be register qubits)
+    uint64_t sup = 0;  // all its qubits
+    TileKind kind = TK_U1;
+    uint32_t q0 = 0, q1 = 0;
+    uint64_t cmask = 0;  // DM: qubits that must all be 1; PERM1: controls
+    cd m[16];
+    uint8_t k0 = 0, k1 = 0;
+};
+
+int quarter(cd c) {
+    if (c == cd(1, 0)) return 0;
+    if (c == cd(0, 1)) return 1;
+    if (c == cd(-1, 0)) return 2;
+    if (c == cd(0, -1)) return 3;
+    return -1;
+}
+
+Cls classify(const LOp &op) {
+    Cls c;
+    const uint32_t k = op.k();
+    for (uint32_t q : op.qubits) c.sup |= 1ull << q;
+    switch (op.kind) {
+    case OpKind::Dense:
+        if (k == 1) {
+            c.tileable = true;
+            c.kind = TK_U1;
+            c.q0 = op.qubits[0];
+            c.act = c.sup;
+            for (int i = 0; i < 4; i++) c.m[i] = op.m[i];
+        } else if (k == 2) {
+            c.tileable = true;
+            c.kind = TK_U2;
+            c.q0 = op.qubits[0];
+            c.q1 = op.qubits[1];
+            c.act = c.sup;
+            for (int i = 0; i < 16; i++) c.m[i] = op.m[i];
+        }
+        return c;
+    case OpKind::Diag: {
+        const uint32_t d = 1u << k;
+        bool mask_form = true;
+        for (uint32_t x = 0; x + 1 < d; x++)
+            if (op.m[x] != cd(1.0, 0.0)) mask_form = false;
+        if (mask_form) {
+            c.tileable = true;
+            c.kind = TK_DM;
+            c.cmask = c.sup;
+            c.m[0] = op.m[d - 1];
+        } else if (k == 1) {
+            c.tileable = true;
+            c.kind = TK_D1R;  // resolved to D1R / D1T per stage
+            c.q0 = op.qubits[0];
+            c.m[0] = op.m[0];
+            c.m[1] = op.m[1];
+        }
+        return c;
+    }
+    case OpKind::Perm: {
+        const uint32_t d = 1u << k;
+        bool all_one = true;
+        for (uint32_t x = 0; x < d; x++)
+            if (op.m[x] != cd(1.0, 0.0)) all_one = false;
+        if (k == 2 && all_one && op.img[0] == 0 && op.img[1] == 2 && op.img[2] == 1 && op.img[3] == 3) {
+            c.tileable = true;
+            c.kind = TK_SWAPR;
+            c.q0 = op.qubits[0];
+            c.q1 = op.qubits[1];
+            c.act = c.sup;
+            return c;
+        }
+        for (uint32_t t = 0; t < k; t++) {
+            const uint32_t tb = 1u << t;
+            bool ok = true;
+            uint32_t cm = ~0u & (d - 1) & ~tb, moved = 0;
+            for (uint32_t x = 0; x < d && ok; x++) {
+                if (op.img[x] == x) {
+                    if (op.m[x] != cd(1.0, 0.0)) ok = false;
+                } else if (op.img[x] == (x ^ tb)) {
+                    cm &= x;
+                    moved++;
+                } else {
+                    ok = false;
+                }
+            }
+            if (!ok || moved == 0) continue;
+            cm &= ~tb;
+            // every state with the controls set moves, with the same phases
+            if (moved != (d >> __builtin_popcount(cm))) continue;
+            const int k0 = quarter(op.m[cm]), k1 = quarter(op.m[cm | tb]);
+            if (k0 < 0 || k1 < 0) continue;
+            bool same = true;
+            for (uint32_t x = 0; x < d; x++)
+                if (op.img[x] != x && op.m[x] != op.m[(x & tb) ? (cm | tb) : cm]) same = false;
+            if (!same) continue;
+            c.tileable = true;
+            c.kind = TK_PERM1;
+            c.q0 = op.qubits[t];
+            c.act = 1ull << c.q0;
+            for (uint32_t j = 0; j < k; j++)
+                if ((cm >> j) & 1) c.cmask |= 1ull << op.qubits[j];
+            c.k0 = (uint8_t)k0;
+            c.k1 = (uint8_t)k1;
+            return c;
+        }
+        return c;
+    }
+    default: return c;
+    }
+}
+
+// The ops of `rem` (in order) that can run when the qubits `avail` may be
+// acted on non-diagonally: those that do not depend on a skipped op.
+void pick(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem, uint64_t avail, std::vector<uint32_t> *out,
+          size_t *count) {
+    uint64_t ba = 0, bn = 0;
+    size_t cnt = 0;
+    for (uint32_t i : rem) {
+        const Cls &c = cl[i];
+        const bool dep = (c.act & ba) || ((c.sup & ~c.act) & bn);
+        if (!dep && (c.act & ~avail) == 0) {
+            cnt++;
+            if (out) out->push_back(i);
+        } else {
+            ba |= c.sup;
+            bn |= c.act;
+        }
+    }
+    if (count) *count = cnt;
+}
+
+// The qubits in the order the remaining ops first need them (non-diagonally).
+std::vector<uint32_t> need_order(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem, uint64_t have) {
+    std::vector<uint32_t> v;
+    uint64_t seen = have;
+    for (uint32_t i : rem) {
+        uint64_t a = cl[i].act & ~seen;
+        while (a) {
+            const int q = __builtin_ctzll(a);
+            a &= a - 1;
+            v.push_back((uint32_t)q);
+            seen |= 1ull << q;
+        }
+    }
+    return v;
+}
+
+// Greedy choice of `cap` qubits (starting from `fixed`, drawn from `pool`)
+// that admit the most ops of `rem`.
+uint64_t choose_set(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem, uint64_t fixed, uint64_t pool,
+                    uint32_t cap) {
+    uint64_t T = fixed;
+    size_t cur = 0;
+    pick(cl, rem, T, nullptr, &cur);
+    while ((uint32_t)__builtin_popcountll(T) < cap) {
+        uint64_t cand = pool & ~T;
+        if (!cand) break;
+        const std::vector<uint32_t> order = need_order(cl, rem, T);
+        int best = -1;
+        size_t bs = cur;
+        // candidates in need order first, so ties go to the earliest needed
+        std::vector<uint32_t> cands;
+        for (uint32_t q : order)
+            if ((cand >> q) & 1) cands.push_back(q);
+        for (uint64_t a = cand; a; a &= a - 1) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(a);
+            if (std::find(cands.begin(), cands.end(), q) == cands.end()) cands.push_back(q);
+        }
+        for (uint32_t q : cands) {
+            size_t s = 0;
+            pick(cl, rem, T | (1ull << q), nullptr, &s);
+            if (s > bs) {
+                bs = s;
+                best = (int)q;
+            }
+        }
+        if (best < 0) best = (int)cands[0];  // no single qubit helps: take the first needed
+        T |= 1ull << best;
+        pick(cl, rem, T, nullptr, &cur);
+    }
+    return T;
+}
+
+// Register bit of a qubit in a stage, or -1.
+int reg_bit(uint32_t q, const uint32_t *pos_of, uint32_t rpos) {
+    const uint32_t p = pos_of[q];
+    if (p == ~0u || !((rpos >> p) & 1)) return -1;
+    return __builtin_popcount(rpos & ((1u << p) - 1));
+}
+
+SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chosen, uint64_t T, uint32_t n,
+                   const TileGeom &geo) {
+    const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, n);
+    // stages over qubits
+    std::vector<std::pair<uint64_t, std::vector<uint32_t>>> st;
+    std::vector<uint32_t> rem = chosen;
+    while (!rem.empty()) {
+        const uint64_t Rq = choose_set(cl, rem, 0, T, R);
+        std::vector<uint32_t> got;
+        pick(cl, rem, Rq, &got, nullptr);
+        if (got.empty()) fail(NCSV_INVALID, "tile planner made no progress");
+        std::vector<uint32_t> left;
+        size_t gi = 0;
+        for (uint32_t i : rem) {
+            if (gi < got.size() && got[gi] == i)
+                gi++;
+            else
+                left.push_back(i);
+        }
+        st.push_back({Rq, got});
+        rem.swap(left);
+    }
+    // positions: low qubits fixed; stage 0's register qubits on the top
+    // (memory-layout register) positions; the rest ascending
+    SweepH sw;
+    sw.tq.assign(L, ~0u);
+    std::vector<uint32_t> pos_of(64, ~0u);
+    for (uint32_t p = 0; p < c; p++) {
+        sw.tq[p] = p;
+        pos_of[p] = p;
+    }
+    uint32_t top = L - R;
+    for (uint64_t a = st.empty() ? 0 : st[0].first; a; a &= a - 1) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        if (pos_of[q] != ~0u || top >= L) continue;
+        sw.tq[top] = q;
+        pos_of[q] = top++;
+    }
+    uint32_t next = c;
+    for (uint64_t a = T; a; a &= a - 1) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        if (pos_of[q] != ~0u) continue;
+        while (next < L && sw.tq[next] != ~0u) next++;
+        sw.tq[next] = q;
+        pos_of[q] = next;
+    }
+    for (uint32_t p = 0; p < L; p++)
+        if (sw.tq[p] == ~0u) fail(NCSV_INVALID, "tile planner left a position empty");
+    // stage layouts and ops
+    uint32_t prev = ((1u << R) - 1) << (L - R);  // memory layout
+    std::vector<int> last(64, -1);               // last op index touching each qubit (U1 merging)
+    for (auto &sg : st) {
+        uint32_t need = 0;
+        for (uint64_t a = sg.first; a; a &= a - 1) need |= 1u << pos_of[__builtin_ctzll(a)];
+        uint32_t rpos = need;
+        // the needed positions, padded from the previous layout, then from the top
+        for (uint32_t p = L; p-- > 0 && (uint32_t)__builtin_popcount(rpos) < R;)
+            if ((prev >> p) & 1) rpos |= 1u << p;
+        for (uint32_t p = L; p-- > 0 && (uint32_t)__builtin_popcount(rpos) < R;) rpos |= 1u << p;
+        StageH S;
+        S.rmask_pos = rpos;
+        S.op_begin = (uint32_t)sw.ops.size();
+        std::fill(last.begin(), last.end(), -1);
+        for (uint32_t i : sg.second) {
+            const Cls &g = cl[i];
+            TileOpH o;
+            o.kind = g.kind;
+            // conditions on the global mask split into register bits and the rest
+            auto split = [&](uint64_t mask) {
+                uint32_t rm = 0;
+                uint64_t gm = 0;
+                for (uint64_t a = mask; a; a &= a - 1) {
+                    const uint32_t q = (uint32_t)__builtin_ctzll(a);
+                    const int b = reg_bit(q, pos_of.data(), rpos);
+                    if (b >= 0)
+                        rm |= 1u << b;
+                    else
+                        gm |= 1ull << q;
+                }
+                o.rmask = rm;
+                o.gmask = gm;
+            };
+            switch (g.kind) {
+            case TK_U1: {
+                o.j0 = (uint8_t)reg_bit(g.q0, pos_of.data(), rpos);
+                for (int t = 0; t < 4; t++) o.m[t] = g.m[t];
+                const int l = last[g.q0];
+                if (l >= 0 && sw.ops[l].kind == TK_U1 && sw.ops[l].j0 == o.j0) {
+                    // merge: later x earlier (FP64 on the host)
+                    cd *e = sw.ops[l].m, r[4];
+                    r[0] = o.m[0] * e[0] + o.m[1] * e[2];
+                    r[1] = o.m[0] * e[1] + o.m[1] * e[3];
+                    r[2] = o.m[2] * e[0] + o.m[3] * e[2];
+                    r[3] = o.m[2] * e[1] + o.m[3] * e[3];
+                    for (int t = 0; t < 4; t++) e[t] = r[t];
+                    continue;
+                }
+                break;
+            }
+            case TK_U2: {
+                int b0 = reg_bit(g.q0, pos_of.data(), rpos), b1 = reg_bit(g.q1, pos_of.data(), rpos);
+                if (b0 < b1) {
+                    for (int t = 0; t < 16; t++) o.m[t] = g.m[t];
+                } else {
+                    // swap the two local bits of the matrix
+                    auto sw2 = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                    for (int a = 0; a < 4; a++)
+                        for (int b = 0; b < 4; b++) o.m[sw2(a) * 4 + sw2(b)] = g.m[a * 4 + b];
+                    std::swap(b0, b1);
+                }
+                o.j0 = (uint8_t)b0;
+                o.j1 = (uint8_t)b1;
+                break;
+            }
+            case TK_SWAPR: {
+                int b0 = reg_bit(g.q0, pos_of.data(), rpos), b1 = reg_bit(g.q1, pos_of.data(), rpos);
+                o.j0 = (uint8_t)std::min(b0, b1);
+                o.j1 = (uint8_t)std::max(b0, b1);
+                break;
+            }
+            case TK_D1R: {
+                const int b = reg_bit(g.q0, pos_of.data(), rpos);
+                o.m[0] = g.m[0];
+                o.m[1] = g.m[1];
+                if (b >= 0) {
+                    o.j0 = (uint8_t)b;
+                } else {
+                    o.kind = TK_D1T;
+                    o.gmask = 1ull << g.q0;
+                }
+                break;
+            }
+            case TK_DM:
+                split(g.cmask);
+                o.m[0] = g.m[0];
+                break;
+            case TK_PERM1:
+                o.j0 = (uint8_t)reg_bit(g.q0, pos_of.data(), rpos);
+                split(g.cmask);
+                o.k0 = g.k0;
+                o.k1 = g.k1;
+                break;
+            default: fail(NCSV_INVALID, "tile planner: unknown op");
+            }
+            for (uint64_t a = g.sup; a; a &= a - 1) last[__builtin_ctzll(a)] = (int)sw.ops.size();
+            sw.ops.push_back(o);
+        }
+        S.op_end = (uint32_t)sw.ops.size();
+        sw.stages.push_back(S);
+        prev = rpos;
+    }
+    sw.gates = (uint32_t)chosen.size();
+    return sw;
+}
+
+std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t> rem, uint32_t n,
+                                const TileGeom &geo) {
+    std::vector<SweepH> out;
+    const uint32_t L = geo.L();
+    const uint64_t fixed = (1ull << std::min(geo.c, n)) - 1;
+    const uint64_t pool = n >= 64 ? ~0ull : (1ull << n) - 1;
+    while (!rem.empty()) {
+        const uint64_t T = choose_set(cl, rem, fixed, pool, L);
+        std::vector<uint32_t> got;
+        pick(cl, rem, T, &got, nullptr);
+        if (got.empty()) fail(NCSV_INVALID, "sweep planner made no progress");
+        out.push_back(build_sweep(cl, got, T, n, geo));
+        std::vector<uint32_t> left;
+        size_t gi = 0;
+        for (uint32_t i : rem) {
+            if (gi < got.size() && got[gi] == i)
+                gi++;
+            else
+                left.push_back(i);
+        }
+        rem.swap(left);
+    }
+    return out;
+}
+
+// A cached plan: the segment's items and their device programs.
+struct Plan {
+    std::vector<SegItem> items;
+    std::vector<size_t> prog_off;  // offset of each tile item's program in dbuf
+    void *dbuf = nullptr;
+    ~Plan() {
+        if (dbuf) (void)hipFree(dbuf);
+    }
+};
+
+}  // namespace
+
+std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const TileGeom &geo) {
+    std::vector<SegItem> items;
+    std::vector<Cls> cl;
+    std::vector<uint32_t> run;
+    auto flush = [&]() {
+        if (run.empty()) return;
+        for (auto &sw : plan_sweeps(cl, run, n, geo)) {
+            SegItem it;
+            it.tile = true;
+            it.sw = std::move(sw);
+            items.push_back(std::move(it));
+        }
+        run.clear();
+    };
+    for (const LOp &op : ops) {
+        if (op.kind == OpKind::Nop) continue;
+        Cls c = classify(op);
+        if (!c.tileable || n < geo.L()) {
+            flush();
+            SegItem it;
+            it.tile = false;
+            it.op = op;
+            items.push_back(std::move(it));
+            continue;
+        }
+        cl.push_back(c);
+        run.push_back((uint32_t)cl.size() - 1);
+    }
+    flush();
+    return items;
+}
+
+void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
+    const TileGeom geo = dev::tile_geom(s->prec);
+    if (s->fusion == 0 || s->n < geo.L()) {
+        for (size_t t = i; t < j; t++) {
+            const COp &op = c->ops[t];
+            if (op.kind != CKind::Gate) continue;
+            dev::apply_lop(s, lower_gate(op.g, op.qubits.data(), (uint32_t)op.qubits.size(), s->n));
+        }
+        return;
+    }
+    const int key = (int)(i * 4 + (size_t)s->prec);
+    std::shared_ptr<Plan> plan;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->plans.find(key);
+        if (it != c->plans.end()) plan = std::static_pointer_cast<Plan>(it->second);
+    }
+    if (!plan) {
+        plan = std::make_shared<Plan>();
+        std::vector<LOp> ops;
+        for (size_t t = i; t < j; t++) {
+            const COp &op = c->ops[t];
+            if (op.kind == CKind::Gate) ops.push_back(lower_gate(op.g, op.qubits.data(), (uint32_t)op.qubits.size(), s->n));
+        }
+        plan->items = plan_segment(ops, s->n, geo);
+        size_t total = 0;
+        for (auto &it : plan->items) {
+            plan->prog_off.push_back(total);
+            if (it.tile) total += dev::tile_prog_bytes(it.sw, s->prec);
+        }
+        if (total) {
+            std::vector<char> host(total);
+            for (size_t k = 0; k < plan->items.size(); k++)
+                if (plan->items[k].tile) dev::tile_prog_pack(plan->items[k].sw, s->prec, host.data() + plan->prog_off[k]);
+            check_hip(hipMalloc(&plan->dbuf, total), "hipMalloc(tile programs)");
+            check_hip(hipMemcpy(plan->dbuf, host.data(), total, hipMemcpyHostToDevice), "upload tile programs");
+        }
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->plans[key] = plan;
+    }
+    for (size_t k = 0; k < plan->items.size(); k++) {
+        const SegItem &it = plan->items[k];
+        if (it.tile)
+            dev::run_tile_sweep(s, it.sw, (const char *)plan->dbuf + plan->prog_off[k], geo);
+        else
+            dev::apply_lop(s, it.op);
+    }
+}
+
+void fuse_release(ncsv_state *) {}
+
+}  // namespace ncsv
+
+// ---------------------------------------------------------------------------
+// Host emulation of the fused schedule (diagnostics): applies a unitary
+// circuit's plan to host amplitudes in FP64 with the kernels' op semantics
+// (register bits per stage, global-index conditions), so the planner can be
+// checked without a device.
+namespace ncsv {
+namespace {
+
+using cdv = std::vector<cd>;
+
+cd ipow_h(int k, cd a) {
+    switch (k & 3) {
+    case 0: return a;
+    case 1: return cd(-a.imag(), a.real());
+    case 2: return cd(-a.real(), -a.imag());
+    default: return cd(a.imag(), -a.real());
+    }
+}
+
+void emu_op(const TileOpH &o, cd *v, uint32_t NR, uint64_t g) {
+    switch (o.kind) {
+    case TK_U1:
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & (1u << o.j0)) continue;
+            const cd x = v[i], y = v[i | (1u << o.j0)];
+            v[i] = o.m[0] * x + o.m[1] * y;
+            v[i | (1u << o.j0)] = o.m[2] * x + o.m[3] * y;
+        }
+        break;
+    case TK_U2:
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & ((1u << o.j0) | (1u << o.j1))) continue;
+            const uint32_t ix[4] = {i, i | (1u << o.j0), i | (1u << o.j1), i | (1u << o.j0) | (1u << o.j1)};
+            cd x[4], y[4];
+            for (int t = 0; t < 4; t++) x[t] = v[ix[t]];
+            for (int a = 0; a < 4; a++) {
+                y[a] = 0;
+                for (int b = 0; b < 4; b++) y[a] += o.m[4 * a + b] * x[b];
+            }
+            for (int t = 0; t < 4; t++) v[ix[t]] = y[t];
+        }
+        break;
+    case TK_D1R:
+        for (uint32_t i = 0; i < NR; i++) v[i] *= ((i >> o.j0) & 1) ? o.m[1] : o.m[0];
+        break;
+    case TK_D1T:
+        for (uint32_t i = 0; i < NR; i++) v[i] *= (g & o.gmask) ? o.m[1] : o.m[0];
+        break;
+    case TK_DM:
+        if ((g & o.gmask) != o.gmask) break;
+        for (uint32_t i = 0; i < NR; i++)
+            if ((i & o.rmask) == o.rmask) v[i] *= o.m[0];
+        break;
+    case TK_PERM1:
+        if ((g & o.gmask) != o.gmask) break;
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & (1u << o.j0)) continue;
+            if ((i & o.rmask) != o.rmask) continue;
+            const cd x = v[i], y = v[i | (1u << o.j0)];
+            v[i] = ipow_h(o.k1, y);
+            v[i | (1u << o.j0)] = ipow_h(o.k0, x);
+        }
+        break;
+    case TK_SWAPR:
+        for (uint32_t i = 0; i < NR; i++)
+            if ((i & (1u << o.j0)) && !(i & (1u << o.j1))) std::swap(v[i], v[(i & ~(1u << o.j0)) | (1u << o.j1)]);
+        break;
+    }
+}
+
+void emu_sweep(const SweepH &sw, cd *amp, uint32_t n, const TileGeom &geo) {
+    const uint32_t L = geo.L(), R = geo.R, NR = 1u << R;
+    uint64_t in = 0;
+    for (uint32_t q : sw.tq) in |= 1ull << q;
+    std::vector<uint32_t> oq;
+    for (uint32_t q = 0; q < n; q++)
+        if (!((in >> q) & 1)) oq.push_back(q);
+    std::vector<cd> loc(size_t(1) << L);
+    for (uint64_t b = 0; b < (1ull << (n - L)); b++) {
+        uint64_t base = 0;
+        for (size_t i = 0; i < oq.size(); i++) base |= ((b >> i) & 1ull) << oq[i];
+        auto gidx = [&](uint32_t t) {
+            uint64_t g = base;
+            for (uint32_t p = 0; p < L; p++)
+                if ((t >> p) & 1) g |= 1ull << sw.tq[p];
+            return g;
+        };
+        for (uint32_t t = 0; t < (1u << L); t++) loc[t] = amp[gidx(t)];
+        for (const StageH &S : sw.stages) {
+            std::vector<uint32_t> rp, tp;
+            for (uint32_t p = 0; p < L; p++) ((S.rmask_pos >> p) & 1 ? rp : tp).push_back(p);
+            for (uint32_t th = 0; th < (1u << tp.size()); th++) {
+                uint32_t tt = 0;
+                for (size_t i = 0; i < tp.size(); i++)
+                    if ((th >> i) & 1) tt |= 1u << tp[i];
+                const uint64_t g = gidx(tt);
+                std::vector<cd> v(NR);
+                std::vector<uint32_t> ti(NR);
+                for (uint32_t r = 0; r < NR; r++) {
+                    uint32_t t = tt;
+                    for (uint32_t j = 0; j < R; j++)
+                        if ((r >> j) & 1) t |= 1u << rp[j];
+                    ti[r] = t;
+                    v[r] = loc[t];
+                }
+                for (uint32_t o = S.op_begin; o < S.op_end; o++) emu_op(sw.ops[o], v.data(), NR, g);
+                for (uint32_t r = 0; r < NR; r++) loc[ti[r]] = v[r];
+            }
+        }
+        for (uint32_t t = 0; t < (1u << L); t++) amp[gidx(t)] = loc[t];
+    }
+}
+
+}  // namespace
+}  // namespace ncsv
+
+extern ncsv_circuit *ncsv_parse_text(const char *text);
+
+extern "C" {
+
+// Diagnostics: the fused schedule of a unitary circuit (gates only) for a
+// state of `prec`, applied on the host to amp (2^n interleaved doubles), and
+// its shape: sweeps, stages, ops, standalone gates.
+ncsv_status ncsv_plan_emulate(const ncsv_circuit *c, ncsv_precision prec, double *amp, uint64_t *stats) {
+    try {
+        using namespace ncsv;
+        const TileGeom geo = dev::tile_geom(prec);
+        std::vector<LOp> ops;
+        for (const COp &op : c->ops) {
+            if (op.kind == CKind::Gate) ops.push_back(lower_gate(op.g, op.qubits.data(), (uint32_t)op.qubits.size(), c->n));
+            else if (op.kind != CKind::Tick) fail(NCSV_INVALID, "plan emulation takes unitary circuits");
+        }
+        const std::vector<SegItem> items = plan_segment(ops, c->n, geo);
+        uint64_t st[4] = {0, 0, 0, 0};
+        cd *a = (cd *)amp;
+        for (const SegItem &it : items) {
+            if (it.tile) {
+                st[0]++;
+                st[1] += it.sw.stages.size();
+                st[2] += it.sw.ops.size();
+                if (a) emu_sweep(it.sw, a, c->n, geo);
+            } else {
+                st[3]++;
+                if (a) fail(NCSV_UNSUPPORTED, "plan emulation: standalone gate in the plan");
+            }
+        }
+        if (stats)
+            for (int i = 0; i < 4; i++) stats[i] = st[i];
+        return NCSV_OK;
+    } catch (ncsv::Error &e) {
+        return e.code;
+    }
+}
+
+}  // extern "C"

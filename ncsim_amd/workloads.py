@@ -1,0 +1,127 @@
+"""Seeded synthetic circuits in ncsim's text format (circuit.rs:4-27), so the
+same file runs on this library and on the reference.
+
+Every generator draws from SplitMix64 and prints angles with repr(), which
+round-trips exactly through the parser.
+"""
+import math
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next_u64(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def gen_f64(self):
+        """[0, 1) with 53 random bits."""
+        return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+
+    def below(self, m):
+        return int(self.gen_f64() * m)
+
+    def shuffle(self, xs):
+        for i in range(len(xs) - 1, 0, -1):
+            j = self.below(i + 1)
+            xs[i], xs[j] = xs[j], xs[i]
+        return xs
+
+
+def random_layers(n, layers, seed, measure=True):
+    """Random Clifford+T layers on n qubits: per layer every qubit gets one
+    of H, S, T, SQRT_X; then CX on a uniformly random perfect matching, each
+    pair kept with probability 1/2; finally M on every qubit."""
+    r = SplitMix64(seed)
+    out = [f"QUBITS {n}"]
+    for _ in range(layers):
+        for q in range(n):
+            out.append(f"{['H', 'S', 'T', 'SQRT_X'][r.below(4)]} {q}")
+        perm = r.shuffle(list(range(n)))
+        for i in range(0, n - 1, 2):
+            if r.gen_f64() < 0.5:
+                out.append(f"CX {perm[i]} {perm[i + 1]}")
+    if measure:
+        out.append("M " + " ".join(str(q) for q in range(n)))
+    return "\n".join(out) + "\n"
+
+
+def rotation_brickwork(n, depth, seed):
+    """Per layer: RX/RY/RZ on every qubit (axis uniform, angle uniform in
+    [0, 2pi)), then CZ on (2i, 2i+1) in even layers and (2i+1, 2i+2) in odd
+    layers."""
+    r = SplitMix64(seed)
+    out = [f"QUBITS {n}"]
+    for layer in range(depth):
+        for q in range(n):
+            ax = ["RX", "RY", "RZ"][r.below(3)]
+            out.append(f"{ax}({r.gen_f64() * 2.0 * math.pi!r}) {q}")
+        for a in range(layer % 2, n - 1, 2):
+            out.append(f"CZ {a} {a + 1}")
+    return "\n".join(out) + "\n"
+
+
+def random_paulis(n, count, seed, max_weight=4):
+    """(x, z, phase) Hermitian Pauli strings: weight uniform 1..max_weight,
+    support a uniform subset, letters uniform over X, Y, Z (Y = i X Z, so the
+    stored phase counts the Ys)."""
+    r = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        w = 1 + r.below(max_weight)
+        sup = r.shuffle(list(range(n)))[:w]
+        x = z = ph = 0
+        for q in sup:
+            letter = r.below(3)
+            if letter in (0, 1):
+                x |= 1 << q
+            if letter in (1, 2):
+                z |= 1 << q
+            if letter == 1:
+                ph += 1
+        out.append((x, z, ph & 3))
+    return out
+
+
+def qft_with_rz(n, rz_count, seed, prefix_x=True):
+    """A seeded X layer (so the controlled phases act), the QFT without the
+    final swaps (H(j), then CPHASE(1, m-j+1) on (m, j) for m > j), and
+    rz_count RZ(theta ~ U[0, 2pi)) gates at uniformly random places."""
+    r = SplitMix64(seed)
+    body = []
+    for j in range(n):
+        body.append(f"H {j}")
+        for m in range(j + 1, n):
+            body.append(f"CPHASE(1,{m - j + 1}) {m} {j}")
+    for _ in range(rz_count):
+        pos = r.below(len(body) + 1)
+        body.insert(pos, f"RZ({r.gen_f64() * 2.0 * math.pi!r}) {r.below(n)}")
+    out = [f"QUBITS {n}"]
+    if prefix_x:
+        for q in range(n):
+            if r.gen_f64() < 0.5:
+                out.append(f"X {q}")
+    return "\n".join(out + body) + "\n"
+
+
+def count_gates(text):
+    """Unfused gate applications of a text circuit (targets grouped by arity)."""
+    ar = {"CX": 2, "CY": 2, "CZ": 2, "SWAP": 2, "CS": 2, "CS_DAG": 2, "CPHASE": 2, "CNOT": 2, "CCZ": 3, "CCX": 3,
+          "TOFFOLI": 3}
+    skip = {"QUBITS", "M", "MX", "MY", "MZ", "MPP", "R", "TICK", "DETECTOR", "OBSERVABLE_INCLUDE", "X_ERROR",
+            "Y_ERROR", "Z_ERROR", "DEPOLARIZE1", "DEPOLARIZE2", "PAULI_CHANNEL_1"}
+    g = 0
+    for line in text.splitlines():
+        line = line.split("#")[0].strip()
+        if not line:
+            continue
+        toks = line.split()
+        name = toks[0].split("(")[0].upper()
+        if name in skip:
+            continue
+        g += (len(toks) - 1) // ar.get(name, 1)
+    return g

@@ -1,0 +1,347 @@
+"""Parity of the HIP library (through its C ABI) with the CPU oracle on the
+same seeded inputs.  Tolerances: 1e-12 max-abs for the complex128 form;
+permutation gates bit-exact; sampling indices identical given the same
+uniforms.  Each test restates a reference test where one exists."""
+import math
+
+import numpy as np
+import pytest
+
+import ncsim_amd as H
+from ncsim_amd import workloads as W
+from oracle import gates as G
+from oracle.circuit import Circuit as OCircuit
+from oracle.exact import ExactState
+from oracle.pauli import Pauli, SimError
+from oracle.statevector import SplitMix64, StateVector
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rand_state(n, seed):
+    rng = np.random.default_rng(seed)
+    a = (rng.random(1 << n) - 0.5) + 1j * (rng.random(1 << n) - 0.5)
+    return a / np.linalg.norm(a)
+
+
+def pair(n, seed):
+    a = rand_state(n, seed)
+    return StateVector(n, a.copy()), H.HipStateVector.from_amplitudes(n, a)
+
+
+def maxerr(o, g):
+    return float(np.abs(o.to_vector() - g.to_vector()).max())
+
+
+NAMED = ["I", "X", "Y", "Z", "H", "S", "S_DAG", "SQRT_X", "SQRT_X_DAG", "SQRT_Y", "SQRT_Y_DAG", "T", "T_DAG",
+         "CX", "CY", "CZ", "SWAP", "CS", "CS_DAG", "CCZ", "CCX"]
+PERMS = {"I", "X", "Y", "Z", "S", "S_DAG", "CX", "CY", "CZ", "SWAP", "CCZ", "CCX"}  # exact phases 0, +-1, +-i
+
+
+def gate_cases():
+    out = [(nm, G.named(nm)) for nm in NAMED]
+    out += [("PHASE", G.Gate("PHASE", k=3, bits=4)), ("PHASE33", G.Gate("PHASE", k=5, bits=33)),
+            ("CPHASE", G.Gate("CPHASE", k=-1, bits=7)), ("RX", G.Gate("RX", theta=0.37)),
+            ("RY", G.Gate("RY", theta=-2.1)), ("RZ", G.Gate("RZ", theta=4.4))]
+    return out
+
+
+@pytest.mark.parametrize("name,g", gate_cases(), ids=[c[0] for c in gate_cases()])
+def test_every_gate_matches_the_oracle(name, g):
+    n = 14
+    rng = np.random.default_rng(abs(hash(name)) % 1000)
+    for trial in range(3):
+        o, d = pair(n, 100 + trial)
+        qs = [int(q) for q in rng.permutation(n)[: g.arity()]]
+        if trial == 0:
+            qs = sorted(qs)
+        elif trial == 1:
+            qs = [n - 1 - i for i in range(g.arity())]  # high qubits, descending
+        o.apply_gate(g, qs)
+        d.apply_gate(g, qs)
+        if name in PERMS:
+            assert np.array_equal(o.to_vector(), d.to_vector()), (name, qs)
+        else:
+            assert maxerr(o, d) < TOL, (name, qs)
+
+
+def test_fiber_kernel_matches_naive_on_high_and_low_qubits():
+    """vector.rs:685-720 with k = 1..6 (k = 6 takes the out-of-place path)."""
+    n = 15
+    o, d = pair(n, 8)
+    rng = np.random.default_rng(8)
+    for trial in range(18):
+        k = 1 + trial % 6
+        qs = []
+        while len(qs) < k:
+            q = int(rng.integers(12, n)) if rng.random() < 0.5 else int(rng.integers(0, 12))
+            if q not in qs:
+                qs.append(q)
+        m = rng.random((1 << k, 1 << k)) + 1j * rng.random((1 << k, 1 << k))
+        o.apply_matrix(m, qs)
+        d.apply_matrix(m, qs)
+        err = maxerr(o, d)
+        scale = np.abs(o.to_vector()).max()
+        assert err < 1e-12 * max(1.0, scale), (trial, qs, err)
+        s = 1.0 / math.sqrt(o.norm_sq().real)
+        o.scale(s)
+        d.scale(s)
+
+
+def test_pauli_expectation_and_projection():
+    """vector.rs:721-756 against the oracle (which is pinned to the dense Pauli action)."""
+    n = 15
+    rng = np.random.default_rng(3)
+    o, d = pair(n, 9)
+    for trial in range(10):
+        kinds = [["I", "X", "Y", "Z"][int(rng.integers(0, 4))] for _ in range(n)]
+        p = Pauli.from_kinds(kinds)
+        if trial % 3 == 0:
+            p = p.negated()
+        assert abs(o.expectation(p) - d.expectation(p)) < TOL
+        u, v = o.clone(), d.clone()
+        u.apply_pauli(p)
+        v.apply_pauli(p)
+        assert np.array_equal(u.to_vector(), v.to_vector())
+        b = bool(rng.integers(0, 2))
+        w1, w2 = o.clone(), d.clone()
+        a1 = w1.project(p, b)
+        a2 = w2.project(p, b)
+        assert abs(a1 - a2) < TOL
+        assert np.array_equal(w1.to_vector(), w2.to_vector())
+    # Z-type projections keep or zero entries exactly
+    z = Pauli.parse("Z3*Z11", n)
+    w1, w2 = o.clone(), d.clone()
+    assert abs(w1.project(z, True) - w2.project(z, True)) < TOL
+    assert np.array_equal(w1.to_vector(), w2.to_vector())
+
+
+def test_batched_expectations_match_single():
+    n = 16
+    o, d = pair(n, 4)
+    ps = [Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 21, 5)]
+    got = d.expectations(ps)
+    for p, g in zip(ps, got):
+        assert abs(o.expectation(p) - g) < TOL
+    terms = [(complex(0.3 * i, -0.1 * i), p) for i, p in enumerate(ps)]
+    assert abs(o.expectation_sum(terms) - d.expectation_sum(terms)) < 1e-11
+
+
+def test_reductions_scalars_and_readout():
+    n = 13
+    o, d = pair(n, 5)
+    assert abs(o.norm_sq() - d.norm_sq()) < TOL
+    o2, d2 = pair(n, 6)
+    assert abs(o.inner(o2) - d.inner(d2)) < TOL
+    for x in (0, 5, (1 << n) - 1):
+        assert o.amplitude(x) == d.amplitude(x)
+        assert abs(o.probability(x) - d.probability(x)) < TOL
+    c = complex(0.3, -1.7)
+    o.scale(c)
+    d.scale(c)
+    assert np.array_equal(o.to_vector(), d.to_vector())
+    o.normalize()
+    d.normalize()
+    assert maxerr(o, d) < TOL
+    # construction
+    p = H.HipStateVector.plus_state(9)
+    assert np.array_equal(p.to_vector(), StateVector.plus_state(9).to_vector())
+    b = H.HipStateVector.basis_state(9, 77)
+    assert np.array_equal(b.to_vector(), StateVector.basis_state(9, 77).to_vector())
+
+
+def test_kron_convention():
+    """tests/backends.rs:181-200."""
+    ca = "QUBITS 3\nH 0 1\nT 0\nCX 0 2\nS 2\nH 2\nCCZ 0 1 2\n"
+    cb = "QUBITS 2\nH 0\nCX 0 1\nT_DAG 1\nH 1\n"
+    va, vb = StateVector.zero_state(3), StateVector.zero_state(2)
+    va.run(OCircuit.parse(ca), SplitMix64(1))
+    vb.run(OCircuit.parse(cb), SplitMix64(1))
+    ga, gb = H.HipStateVector.zero_state(3), H.HipStateVector.zero_state(2)
+    ga.run(H.Circuit.parse(ca))
+    gb.run(H.Circuit.parse(cb))
+    want = va.kron(vb).to_vector()
+    got = ga.tensor(gb).to_vector()
+    assert np.abs(want - got).max() < TOL
+    t = H.HipStateVector.tensor_all([ga, gb, H.HipStateVector.plus_state(1)])
+    assert t.num_qubits() == 6 and abs(t.norm_sq() - 1) < TOL
+
+
+def test_sampling_matches_the_oracle_given_the_same_uniforms():
+    """vector.rs:767-800 (Born rule, 200k shots, never a zero-weight index) and
+    identical indices against the oracle for the same draws."""
+    n = 14
+    i = np.arange(1 << n)
+    amp = np.where(i % 4099 == 7, 3.0, 0.01 * (i % 13)).astype(complex)
+    o = StateVector(n, amp)
+    d = H.HipStateVector.from_amplitudes(n, amp)
+    us = np.random.default_rng(2).random(200_000)
+    so = o.sample_with_uniforms(us)
+    sd = d.sample_with_uniforms(us)
+    assert np.mean(so == sd) > 0.9999
+    total = np.sum(np.abs(amp) ** 2)
+    counts = np.bincount(sd.astype(np.int64), minlength=1 << n)
+    for j in range(7, 1 << n, 4099):
+        p = abs(amp[j]) ** 2 / total
+        assert abs(counts[j] / len(us) - p) < 6 * math.sqrt(p * (1 - p) / len(us))
+    assert np.all((sd % 13 != 0) | (sd % 4099 == 7))
+    # a vanished state is ZeroState
+    z = H.HipStateVector.zero_state(3)
+    z.project(Pauli.parse("Z0", 3), True)
+    with pytest.raises(H.NcsvError) as e:
+        z.sample_with_uniforms([0.5])
+    assert e.value.kind == "ZeroState"
+
+
+def test_ghz_and_clifford_t_against_the_exact_oracle():
+    """vector.rs:580-633 and tests/backends.rs:18-75."""
+    c = OCircuit(3)
+    c.gate("H", [0]).gate("CX", [0, 1]).gate("CX", [1, 2])
+    d = H.HipStateVector.zero_state(3)
+    d.run(H.Circuit.parse(str(c)))
+    e = ExactState(3).run(c)
+    assert np.abs(np.array(e.to_vector()) - d.to_vector()).max() < 1e-15
+    assert abs(d.expectation(Pauli.parse("X0*X1*X2")) - 1) < 1e-15
+    assert d.expectation(Pauli.parse("Z0", 3)) == 0
+    w = d.clone().project(Pauli.parse("Z0", 3), True)
+    assert abs(w.real - 0.5) < 2e-16
+    from tests.test_oracle import CLIFFORD_T
+
+    oc = OCircuit.parse(CLIFFORD_T)
+    d = H.HipStateVector.zero_state(4)
+    d.run(H.Circuit.parse(CLIFFORD_T))
+    e = ExactState(4).run(oc)
+    assert np.abs(np.array(e.to_vector()) - d.to_vector()).max() < 1e-14
+    for text in ["X0*Z1", "Y2", "Z0*Z1*Z2*Z3", "X3"]:
+        p = Pauli.parse(text, 4)
+        assert abs(d.expectation(p) - e.expectation(p)) < 1e-12
+
+
+def run_both(text, seed, fusion=None):
+    o = StateVector.zero_state(OCircuit.parse(text).n)
+    ro = o.run(OCircuit.parse(text), SplitMix64(seed))
+    d = H.HipStateVector.zero_state(o.n)
+    if fusion is not None:
+        d.set_fusion(fusion)
+    rd = d.run(H.Circuit.parse(text), SplitMix64(seed))
+    return o, d, ro, rd
+
+
+def test_measurements_resets_and_noise_follow_the_same_draws():
+    """mod.rs:113-165: the records and final states agree when both sides
+    draw from the same generator."""
+    text = ("QUBITS 5\nH 0 1 2 3 4\nT 1\nCX 0 1 2 3\nM 0\nMPP X1*X2 Z3*Z4\nR 2\nDEPOLARIZE1(0.3) 0 1 2 3 4\n"
+            "RY(0.4) 3\nMY 3\nDEPOLARIZE2(0.5) 0 4 1 2\nX_ERROR(0.5) 3\nM 1 2 3 4\n")
+    for seed in range(6):
+        o, d, ro, rd = run_both(text, seed)
+        assert ro == rd, seed
+        assert maxerr(o, d) < TOL
+
+
+def test_measured_clifford_circuit():
+    """tests/backends.rs:77-107."""
+    text = "QUBITS 3\nH 0\nCX 0 1\nCX 1 2\nS 2\nMPP Z0*Z1\nMPP X0*X1*X2\nM 0\n"
+    o, d, ro, rd = run_both(text, 3)
+    assert rd[0] is False and ro == rd
+    z0 = Pauli.parse("Z0", 3)
+    sign = -1.0 if rd[2] else 1.0
+    assert abs(d.expectation(z0).real / d.norm_sq().real - sign) < 1e-12
+
+
+@pytest.mark.parametrize("fusion", [0, 1])
+def test_random_clifford_t_layers_exact_at_12_qubits(fusion):
+    """The 20-qubit random-layer workload's family at 12 qubits and depth 30,
+    against the exact Z[w] simulator."""
+    text = W.random_layers(12, 30, 7, measure=False)
+    e = ExactState(12).run(OCircuit.parse(text))
+    d = H.HipStateVector.zero_state(12)
+    d.set_fusion(fusion)
+    d.run(H.Circuit.parse(text))
+    assert np.abs(np.array(e.to_vector()) - d.to_vector()).max() < 1e-12
+
+
+@pytest.mark.parametrize("fusion", [0, 1])
+def test_random_layers_20_qubits_with_measurements(fusion):
+    """20 qubits, 200 layers of {H,S,T,SQRT_X} + random CX matching, then M
+    on all qubits and 1000 shots: amplitudes before measuring, the record and
+    the shots, against the oracle."""
+    text = W.random_layers(20, 200, 11, measure=False)
+    o, d, _, _ = run_both(text, 1, fusion)
+    assert maxerr(o, d) < TOL
+    us = np.random.default_rng(0).random(1000)
+    assert np.array_equal(o.sample_with_uniforms(us), d.sample_with_uniforms(us))
+    meas = "QUBITS 20\nM " + " ".join(map(str, range(20))) + "\n"
+    ro = o.run(OCircuit.parse(meas), SplitMix64(5))
+    rd = d.run(H.Circuit.parse(meas), SplitMix64(5))
+    assert ro == rd
+
+
+@pytest.mark.parametrize("fusion", [0, 1])
+def test_rotation_brickwork_22_qubits_and_64_paulis(fusion):
+    """The 30-qubit brickwork's family (RX/RY/RZ + CZ, depth 24) at 22 qubits:
+    amplitudes and 64 Pauli expectations within 1e-12 of the oracle."""
+    n = 22
+    text = W.rotation_brickwork(n, 24, 3)
+    o, d, _, _ = run_both(text, 0, fusion)
+    assert maxerr(o, d) < TOL
+    ps = [Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 64, 4)]
+    got = d.expectations(ps)
+    for p, g in zip(ps, got):
+        assert abs(o.expectation(p) - g) < TOL
+
+
+def test_qft_phase_bits_beyond_32():
+    """CPHASE(1, bits) up to bits = 24 here (the reference stops at 32 bits,
+    phasepoly.rs:550; the library takes 62): against the oracle."""
+    text = W.qft_with_rz(24, 50, 2)
+    o, d, _, _ = run_both(text, 0)
+    assert maxerr(o, d) < TOL
+    # the closed form of one 33-bit phase
+    s = H.HipStateVector.zero_state(2)
+    s.apply_gate("X", [0])
+    s.apply_gate("X", [1])
+    s.apply_gate(H.Gate("CPHASE", k=1, bits=33), [0, 1])
+    a = s.amplitude(3)
+    assert abs(a - complex(math.cos(2 * math.pi / 2 ** 33), math.sin(2 * math.pi / 2 ** 33))) < 1e-16
+
+
+def test_errors_follow_the_reference():
+    d = H.HipStateVector.zero_state(3)
+    for g, qs, kind in [("CX", [0, 0], "DuplicateQubit"), ("H", [3], "QubitOutOfRange"), ("CX", [0], "Arity")]:
+        with pytest.raises(H.NcsvError) as e:
+            d.apply_gate(g, qs)
+        assert e.value.kind == kind
+    with pytest.raises(H.NcsvError) as e:
+        d.project(Pauli(3, 1, 0, 1), False)
+    assert e.value.kind == "Invalid"
+    with pytest.raises(H.NcsvError) as e:
+        d.expectation(Pauli(2, 1, 0, 0))
+    assert e.value.kind == "Invalid"
+    w = d.clone()
+    w.project(Pauli.parse("Z0", 3), True)
+    with pytest.raises(H.NcsvError) as e:
+        w.normalize()
+    assert e.value.kind == "ZeroState"
+    with pytest.raises(H.NcsvError) as e:
+        d.run(H.Circuit.parse("QUBITS 4\nH 0\n"))
+    assert e.value.kind == "Invalid"
+
+
+def test_complex64_form_tracks_complex128():
+    """The complex64 form (FP32 amplitudes, FP64 reductions and scale): its
+    fidelity with the complex128 form on a 20-qubit brickwork and after
+    measurements."""
+    n = 20
+    text = W.rotation_brickwork(n, 12, 9) + "M 0 1 2\n"
+    a = H.HipStateVector.zero_state(n, H.C128)
+    b = H.HipStateVector.zero_state(n, H.C64)
+    ra = a.run(H.Circuit.parse(text), SplitMix64(4))
+    rb = b.run(H.Circuit.parse(text), SplitMix64(4))
+    assert ra == rb
+    va, vb = a.to_vector(), b.to_vector()
+    fid = abs(np.vdot(va, vb)) ** 2 / (np.vdot(va, va).real * np.vdot(vb, vb).real)
+    assert 1 - fid < 1e-5
+    assert np.abs(va - vb).max() < 1e-5
+    assert abs(a.norm_sq() - b.norm_sq()) < 1e-6
