@@ -208,6 +208,64 @@ int reg_bit(uint32_t q, const uint32_t *pos_of, uint32_t rpos) {
     return __builtin_popcount(rpos & ((1u << p) - 1));
 }
 
+// Peephole pass over a sweep's ops: one-qubit diagonals diag(d0, d1) become
+// a global phase d0 (folded into one dense op, which touches every
+// amplitude) times a phase d1/d0 on the amplitudes with the bit set; 2x2
+// matrices with real entries, or with a real diagonal and an imaginary
+// off-diagonal, get their cheaper forms.
+void finish_sweep(SweepH *sw) {
+    cd gphase(1.0, 0.0);
+    for (TileOpH &o : sw->ops) {
+        if (o.kind == TK_D1R || o.kind == TK_D1T) {
+            const cd d0 = o.m[0], d1 = o.m[1];
+            if (d0 != cd(1.0, 0.0)) gphase *= d0;
+            const cd ph = d0 == cd(1.0, 0.0) ? d1 : d1 / d0;
+            if (o.kind == TK_D1R) {
+                o.rmask = 1u << o.j0;
+                o.gmask = 0;
+            } else {
+                o.rmask = 0;
+            }
+            o.kind = TK_DM;
+            o.m[0] = ph;
+        }
+    }
+    if (gphase != cd(1.0, 0.0)) {
+        bool folded = false;
+        for (TileOpH &o : sw->ops) {
+            if (o.kind == TK_U1 || o.kind == TK_U2) {
+                const int nm = o.kind == TK_U1 ? 4 : 16;
+                for (int t = 0; t < nm; t++) o.m[t] *= gphase;
+                folded = true;
+                break;
+            }
+        }
+        if (!folded) {
+            // a phase on every amplitude: DM with empty masks, first op
+            TileOpH g;
+            g.kind = TK_DM;
+            g.m[0] = gphase;
+            sw->ops.insert(sw->ops.begin(), g);
+            for (size_t s = 0; s < sw->stages.size(); s++) {
+                if (s > 0) sw->stages[s].op_begin++;
+                sw->stages[s].op_end++;
+            }
+        }
+    }
+    for (TileOpH &o : sw->ops) {
+        if (o.kind != TK_U1) continue;
+        bool real = true, xform = true;
+        for (int t = 0; t < 4; t++) {
+            if (o.m[t].imag() != 0.0) real = false;
+            if ((t == 0 || t == 3) ? o.m[t].imag() != 0.0 : o.m[t].real() != 0.0) xform = false;
+        }
+        if (real)
+            o.kind = TK_U1R;
+        else if (xform)
+            o.kind = TK_U1X;
+    }
+}
+
 SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chosen, uint64_t T, uint32_t n,
                    const TileGeom &geo) {
     const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, n);
@@ -360,6 +418,7 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
         prev = rpos;
     }
     sw.gates = (uint32_t)chosen.size();
+    finish_sweep(&sw);
     return sw;
 }
 
@@ -442,7 +501,7 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         }
         return;
     }
-    const int key = (int)(i * 4 + (size_t)s->prec);
+    const int64_t key = ((int64_t)i << 20) | ((int64_t)geo.key() << 2) | (int64_t)s->prec;
     std::shared_ptr<Plan> plan;
     {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -504,6 +563,8 @@ cd ipow_h(int k, cd a) {
     }
 }
 
+uint64_t gmask_all(const TileOpH &) { return 0; }
+
 void emu_op(const TileOpH &o, cd *v, uint32_t NR, uint64_t g) {
     switch (o.kind) {
     case TK_U1:
@@ -514,6 +575,13 @@ void emu_op(const TileOpH &o, cd *v, uint32_t NR, uint64_t g) {
             v[i | (1u << o.j0)] = o.m[2] * x + o.m[3] * y;
         }
         break;
+    case TK_U1R:
+    case TK_U1X: {
+        TileOpH g = o;
+        g.kind = TK_U1;
+        emu_op(g, v, NR, gmask_all(g));
+        break;
+    }
     case TK_U2:
         for (uint32_t i = 0; i < NR; i++) {
             if (i & ((1u << o.j0) | (1u << o.j1))) continue;

@@ -26,6 +26,8 @@ enum TileKind : uint8_t {
     TK_DM = 4,     // phase m[0] where (g & gmask) == gmask and (r & rmask) == rmask
     TK_PERM1 = 5,  // controlled flip of register bit j0: new1 = i^k0 v0, new0 = i^k1 v1
     TK_SWAPR = 6,  // swap register bits j0 < j1
+    TK_U1R = 7,    // 2x2 with real entries (H, RY, ...)
+    TK_U1X = 8,    // 2x2 with a real diagonal and an imaginary off-diagonal (RX, ...)
 };
 
 struct TileOpH {
@@ -55,7 +57,9 @@ struct SegItem {
 
 struct TileGeom {
     uint32_t LT = 8, R = 4, c = 4;  // threads = 2^LT, registers per thread = 2^R, fixed low qubits
+    bool pipe = false;              // load the next op while applying the current one
     uint32_t L() const { return LT + R; }
+    int key() const { return (int)((LT << 12) | (R << 8) | (c << 4) | (pipe ? 1 : 0)); }
 };
 
 // The fused schedule of a segment of lowered gates on n qubits.

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "ncsv_internal.hpp"
 
@@ -383,15 +384,17 @@ __global__ __launch_bounds__(TPB) void k_project_pair(typename VecOf<T>::type *a
 // ------------------------------------------------------------ misc sweeps
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_scale(typename VecOf<T>::type *a, uint64_t N, T cr, T ci) {
+    // products round separately, as in the reference's complex multiply
+#pragma clang fp contract(off)
     using V = typename VecOf<T>::type;
     const uint64_t i = gtid();
     if (i >= N) return;
     const V v = a[i];
     if (czero(v)) return;
-    V c;
-    c.x = cr;
-    c.y = ci;
-    a[i] = cmulv(v, c);
+    V r;
+    r.x = v.x * cr - v.y * ci;
+    r.y = v.x * ci + v.y * cr;
+    a[i] = r;
 }
 
 template <typename T>
@@ -845,9 +848,20 @@ double norm_sq(ncsv_state *s) {
     return out[0];
 }
 
+bool expectations_tiled(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
+                        double *out);
+
 void expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
                   double *out) {
     const uint64_t mask = s->n < 64 ? (1ull << s->n) - 1 : ~0ull;
+    {
+        std::vector<uint64_t> xm(count), zm(count);
+        for (uint32_t i = 0; i < count; i++) {
+            xm[i] = x[i] & mask;
+            zm[i] = z[i] & mask;
+        }
+        if (count > 1 && expectations_tiled(s, count, xm.data(), zm.data(), ph, out)) return;
+    }
     for (uint32_t p0 = 0; p0 < count; p0 += MAXP) {
         RedArgs r;
         memset(&r, 0, sizeof(r));

@@ -107,7 +107,7 @@ struct ncsv_circuit {
     uint64_t measurements = 0;
     // cached fused schedule, keyed by the state's precision (built on demand)
     std::mutex mu;
-    std::map<int, std::shared_ptr<void>> plans;
+    std::map<int64_t, std::shared_ptr<void>> plans;
 };
 
 namespace ncsv {

@@ -11,7 +11,9 @@
 // controls test the thread's global index bits.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "fuse.hpp"
 
@@ -26,11 +28,13 @@ struct StageD {
     uint32_t rpos, op_begin, op_end, pad;
 };
 
-template <typename T> struct alignas(16) TileOpD {
+// One op: 24-byte header and 8 coefficients (U2 keeps its 16 complex
+// entries in a side array at index xidx).
+template <typename T> struct alignas(8) TileOpD {
     uint8_t kind, j0, j1, k0, k1, pad0, pad1, pad2;
-    uint32_t rmask, pad3;
+    uint32_t rmask, xidx;
     uint64_t gmask;
-    T m[32];  // interleaved complex
+    T m[8];
 };
 
 struct SweepArgs {
@@ -140,6 +144,40 @@ __device__ __forceinline__ void u1(V (&v)[NR], const T *m) {
     }
 }
 
+// real 2x2 [[a, b], [c, d]]
+template <int J, int NR, typename V, typename T>
+__device__ __forceinline__ void u1r(V (&v)[NR], T a, T b, T c, T d) {
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        if (i & (1 << J)) continue;
+        const V x = v[i], y = v[i | (1 << J)];
+        V p, q;
+        p.x = a * x.x + b * y.x;
+        p.y = a * x.y + b * y.y;
+        q.x = c * x.x + d * y.x;
+        q.y = c * x.y + d * y.y;
+        v[i] = p;
+        v[i | (1 << J)] = q;
+    }
+}
+
+// [[a, i b], [i c, d]] with a, b, c, d real
+template <int J, int NR, typename V, typename T>
+__device__ __forceinline__ void u1x(V (&v)[NR], T a, T b, T c, T d) {
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        if (i & (1 << J)) continue;
+        const V x = v[i], y = v[i | (1 << J)];
+        V p, q;
+        p.x = a * x.x - b * y.y;
+        p.y = a * x.y + b * y.x;
+        q.x = d * y.x - c * x.y;
+        q.y = d * y.y + c * x.x;
+        v[i] = p;
+        v[i | (1 << J)] = q;
+    }
+}
+
 template <int A, int B, int NR, typename V, typename T>
 __device__ __forceinline__ void u2(V (&v)[NR], const T *m) {
     T mm[32];
@@ -219,10 +257,11 @@ template <int A, int B, int NR, typename V> __device__ __forceinline__ void swap
     case 4: if constexpr (R > 4) FN(4); break;
 
 template <typename T, int R>
-__device__ __forceinline__ void apply_op(const TileOpD<T> *__restrict__ opp, typename VecOf<T>::type (&v)[1 << R],
-                                         uint64_t g) {
+__device__ __forceinline__ void apply_op(const TileOpD<T> &op, typename VecOf<T>::type (&v)[1 << R], uint64_t g,
+                                         const T *__restrict__ u2m) {
     using V = typename VecOf<T>::type;
     constexpr int NR = 1 << R;
+    const TileOpD<T> *opp = &op;
     const int kind = opp->kind;
     const int j0 = opp->j0;
     switch (kind) {
@@ -232,8 +271,23 @@ __device__ __forceinline__ void apply_op(const TileOpD<T> *__restrict__ opp, typ
 #undef U1C
         break;
     }
+    case TK_U1R: {
+        const T a = opp->m[0], b = opp->m[1], c = opp->m[2], d = opp->m[3];
+#define U1RC(J) u1r<J, NR>(v, a, b, c, d)
+        switch (j0) { BIT_CASES(U1RC) }
+#undef U1RC
+        break;
+    }
+    case TK_U1X: {
+        const T a = opp->m[0], b = opp->m[1], c = opp->m[2], d = opp->m[3];
+#define U1XC(J) u1x<J, NR>(v, a, b, c, d)
+        switch (j0) { BIT_CASES(U1XC) }
+#undef U1XC
+        break;
+    }
     case TK_U2: {
-#define U2C(A, B) u2<A, B, NR>(v, opp->m)
+        const T *m2 = u2m + 32 * (size_t)opp->xidx;
+#define U2C(A, B) u2<A, B, NR>(v, m2)
         switch (j0 * 8 + opp->j1) { PAIR_CASES(U2C) }
 #undef U2C
         break;
@@ -289,10 +343,10 @@ __device__ __forceinline__ void apply_op(const TileOpD<T> *__restrict__ opp, typ
     }
 }
 
-template <typename T, int LT, int R>
+template <typename T, int LT, int R, bool PIPE>
 __global__ __launch_bounds__(1 << LT) void k_tile(typename VecOf<T>::type *__restrict__ amp, SweepArgs a,
                                                   const StageD *__restrict__ st,
-                                                  const TileOpD<T> *__restrict__ ops) {
+                                                  const TileOpD<T> *__restrict__ ops, const T *__restrict__ u2m) {
     using V = typename VecOf<T>::type;
     constexpr int L = LT + R, NR = 1 << R;
     __shared__ V lds[1 << L];
@@ -337,7 +391,19 @@ __global__ __launch_bounds__(1 << LT) void k_tile(typename VecOf<T>::type *__res
                 }
             }
         }
-        for (uint32_t o = S.op_begin; o < S.op_end; o++) apply_op<T, R>(ops + o, v, g);
+        // PIPE: the next op's record is loaded while the current one is applied
+        const uint32_t cnt = S.op_end - S.op_begin;
+        if (!PIPE) {
+            for (uint32_t o = S.op_begin; o < S.op_end; o++) apply_op<T, R>(ops[o], v, g, u2m);
+        } else if (cnt) {
+            const TileOpD<T> *op = ops + S.op_begin;
+            TileOpD<T> nxt = op[0];
+            for (uint32_t o = 0; o < cnt; o++) {
+                const TileOpD<T> cur = nxt;
+                if (o + 1 < cnt) nxt = op[o + 1];
+                apply_op<T, R>(cur, v, g, u2m);
+            }
+        }
     }
     if (cur != memr) exchange<V, LT, R>(lds, v, tid, cur, memr);
 #pragma unroll
@@ -352,12 +418,22 @@ __global__ __launch_bounds__(1 << LT) void k_tile(typename VecOf<T>::type *__res
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-template <typename T> size_t prog_bytes_t(const SweepH &sw) {
-    return align16(sw.stages.size() * sizeof(StageD)) + sw.ops.size() * sizeof(TileOpD<T>);
+size_t n_u2(const SweepH &sw) {
+    size_t k = 0;
+    for (auto &o : sw.ops) k += o.kind == TK_U2;
+    return k;
 }
+
+// program: stages | ops | U2 coefficients
+template <typename T> size_t ops_off(const SweepH &sw) { return align16(sw.stages.size() * sizeof(StageD)); }
+template <typename T> size_t u2_off(const SweepH &sw) {
+    return ops_off<T>(sw) + align16(sw.ops.size() * sizeof(TileOpD<T>));
+}
+template <typename T> size_t prog_bytes_t(const SweepH &sw) { return u2_off<T>(sw) + n_u2(sw) * 32 * sizeof(T); }
 
 template <typename T> void pack_t(const SweepH &sw, void *dst) {
     char *p = (char *)dst;
+    memset(dst, 0, prog_bytes_t<T>(sw));
     for (size_t s = 0; s < sw.stages.size(); s++) {
         StageD d;
         d.rpos = sw.stages[s].rmask_pos;
@@ -366,7 +442,9 @@ template <typename T> void pack_t(const SweepH &sw, void *dst) {
         d.pad = 0;
         memcpy(p + s * sizeof(StageD), &d, sizeof(d));
     }
-    p += align16(sw.stages.size() * sizeof(StageD));
+    char *po = (char *)dst + ops_off<T>(sw);
+    T *pu = (T *)((char *)dst + u2_off<T>(sw));
+    uint32_t nu = 0;
     for (size_t o = 0; o < sw.ops.size(); o++) {
         const TileOpH &h = sw.ops[o];
         TileOpD<T> d;
@@ -378,15 +456,42 @@ template <typename T> void pack_t(const SweepH &sw, void *dst) {
         d.k1 = h.k1;
         d.rmask = h.rmask;
         d.gmask = h.gmask;
-        for (int t = 0; t < 16; t++) {
-            d.m[2 * t] = (T)h.m[t].real();
-            d.m[2 * t + 1] = (T)h.m[t].imag();
+        switch (h.kind) {
+        case TK_U1:
+            for (int t = 0; t < 4; t++) {
+                d.m[2 * t] = (T)h.m[t].real();
+                d.m[2 * t + 1] = (T)h.m[t].imag();
+            }
+            break;
+        case TK_U1R:
+            for (int t = 0; t < 4; t++) d.m[t] = (T)h.m[t].real();
+            break;
+        case TK_U1X:
+            d.m[0] = (T)h.m[0].real();
+            d.m[1] = (T)h.m[1].imag();
+            d.m[2] = (T)h.m[2].imag();
+            d.m[3] = (T)h.m[3].real();
+            break;
+        case TK_U2:
+            d.xidx = nu;
+            for (int t = 0; t < 16; t++) {
+                pu[32 * nu + 2 * t] = (T)h.m[t].real();
+                pu[32 * nu + 2 * t + 1] = (T)h.m[t].imag();
+            }
+            nu++;
+            break;
+        default:
+            for (int t = 0; t < 2; t++) {
+                d.m[2 * t] = (T)h.m[t].real();
+                d.m[2 * t + 1] = (T)h.m[t].imag();
+            }
+            break;
         }
-        memcpy(p + o * sizeof(d), &d, sizeof(d));
+        memcpy(po + o * sizeof(d), &d, sizeof(d));
     }
 }
 
-template <typename T, int LT, int R> void launch_t(ncsv_state *s, const SweepH &sw, const void *dprog) {
+template <typename T, int LT, int R, bool PIPE> void launch_t(ncsv_state *s, const SweepH &sw, const void *dprog) {
     using V = typename VecOf<T>::type;
     constexpr int L = LT + R;
     SweepArgs a;
@@ -402,11 +507,12 @@ template <typename T, int LT, int R> void launch_t(ncsv_state *s, const SweepH &
     for (uint32_t q = 0; q < s->n; q++)
         if (!((in >> q) & 1)) a.oq[k++] = (uint8_t)q;
     const StageD *st = (const StageD *)dprog;
-    const TileOpD<T> *ops = (const TileOpD<T> *)((const char *)dprog + align16(sw.stages.size() * sizeof(StageD)));
+    const TileOpD<T> *ops = (const TileOpD<T> *)((const char *)dprog + ops_off<T>(sw));
+    const T *u2m = (const T *)((const char *)dprog + u2_off<T>(sw));
     const uint64_t tiles = 1ull << (s->n - L);
     KTimer kt(s, "tile", 2.0 * (double)s->amp_bytes());
-    hipLaunchKernelGGL((k_tile<T, LT, R>), dim3((uint32_t)tiles), dim3(1 << LT), 0, s->stream, (V *)s->amp, a, st,
-                       ops);
+    hipLaunchKernelGGL((k_tile<T, LT, R, PIPE>), dim3((uint32_t)tiles), dim3(1 << LT), 0, s->stream, (V *)s->amp, a, st,
+                       ops, u2m);
     check_hip(hipGetLastError(), "k_tile");
 }
 
@@ -416,7 +522,8 @@ namespace dev {
 
 // complex128: 256 threads x 16 amplitudes (64 KiB of LDS), 16 contiguous
 // amplitudes (256 B) per run.  complex64: 256 threads x 32 amplitudes
-// (64 KiB), 32 contiguous (256 B).
+// (64 KiB), 32 contiguous (256 B).  NCSV_TILE_LT / NCSV_TILE_R / NCSV_TILE_C
+// select another built shape (see run_tile_sweep).
 TileGeom tile_geom(ncsv_precision prec) {
     TileGeom g;
     if (prec == NCSV_C128) {
@@ -428,6 +535,10 @@ TileGeom tile_geom(ncsv_precision prec) {
         g.R = 5;
         g.c = 5;
     }
+    if (const char *e = getenv("NCSV_TILE_LT")) g.LT = (uint32_t)atoi(e);
+    if (const char *e = getenv("NCSV_TILE_R")) g.R = (uint32_t)atoi(e);
+    if (const char *e = getenv("NCSV_TILE_C")) g.c = (uint32_t)atoi(e);
+    if (const char *e = getenv("NCSV_TILE_PIPE")) g.pipe = atoi(e) != 0;
     return g;
 }
 
@@ -443,14 +554,246 @@ void tile_prog_pack(const SweepH &sw, ncsv_precision prec, void *dst) {
 }
 
 void run_tile_sweep(ncsv_state *s, const SweepH &sw, const void *dprog, const TileGeom &geo) {
-    if (s->prec == NCSV_C128) {
-        if (geo.LT != 8 || geo.R != 4) fail(NCSV_INVALID, "unsupported tile shape");
-        launch_t<double, 8, 4>(s, sw, dprog);
-    } else {
-        if (geo.LT != 8 || geo.R != 5) fail(NCSV_INVALID, "unsupported tile shape");
-        launch_t<float, 8, 5>(s, sw, dprog);
+#define TILE_SHAPE(T, LT_, R_)                                   \
+    if (geo.LT == LT_ && geo.R == R_) {                          \
+        if (geo.pipe)                                            \
+            launch_t<T, LT_, R_, true>(s, sw, dprog);            \
+        else                                                     \
+            launch_t<T, LT_, R_, false>(s, sw, dprog);           \
+        s->sweeps++;                                             \
+        return;                                                  \
     }
-    s->sweeps++;
+    if (s->prec == NCSV_C128) {
+        TILE_SHAPE(double, 8, 4)
+        TILE_SHAPE(double, 9, 4)
+        TILE_SHAPE(double, 9, 3)
+    } else {
+        TILE_SHAPE(float, 8, 5)
+        TILE_SHAPE(float, 9, 5)
+        TILE_SHAPE(float, 9, 4)
+    }
+#undef TILE_SHAPE
+    fail(NCSV_INVALID, "unsupported tile shape");
+}
+
+}  // namespace dev
+}  // namespace ncsv
+
+// ---------------------------------------------------------------------------
+// Expectation values of a group of Paulis whose X parts lie inside one tile:
+// each workgroup reads its tile once (to registers and LDS); for every Pauli
+// the partner amplitude psi(c ^ x) comes from LDS.  Sums are FP64.
+namespace ncsv {
+namespace {
+
+constexpr int EXP_MAX = 16;  // Paulis per group
+
+struct ExpArgs {
+    SweepArgs a;
+    uint32_t count, pad;
+};
+
+// i^k * a with selects instead of branches (exact)
+template <typename V> __device__ __forceinline__ V ipow_sel(int k, V a) {
+    V r;
+    const bool odd = k & 1, neg = k & 2;
+    r.x = odd ? -a.y : a.x;
+    r.y = odd ? a.x : a.y;
+    if (neg) {
+        r.x = -r.x;
+        r.y = -r.y;
+    }
+    return r;
+}
+
+// per-group Pauli parameters, in device memory (uniform loads)
+struct ExpParams {
+    uint64_t z[EXP_MAX];     // Z part on qubits
+    uint32_t xloc[EXP_MAX];  // X part on tile positions
+    uint32_t ph[EXP_MAX];
+};
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <typename T, int LT, int R>
+__global__ __launch_bounds__(1 << LT) void k_expect_tile(const typename VecOf<T>::type *__restrict__ amp, ExpArgs e,
+                                                         const ExpParams *__restrict__ P, double *part) {
+    using V = typename VecOf<T>::type;
+    constexpr int L = LT + R, NR = 1 << R, NT = 1 << LT;
+    __shared__ V lds[1 << L];
+    __shared__ double red[NT / 64][2 * EXP_MAX];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    uint64_t base = 0;
+    const int nout = (int)e.a.n - L;
+    for (int i = 0; i < nout; i++) base |= ((b >> i) & 1ull) << e.a.oq[i];
+    uint64_t gmem = base;
+#pragma unroll
+    for (int p = 0; p < LT; p++) gmem |= (uint64_t)((tid >> p) & 1u) << e.a.tq[p];
+    uint64_t rq[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) rq[j] = 1ull << e.a.tq[LT + j];
+    V v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        uint64_t o = gmem;
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            if ((r >> j) & 1) o |= rq[j];
+        v[r] = amp[o];
+        lds[swz<V>(tid | ((uint32_t)r << LT))] = v[r];
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    const uint32_t cnt = e.count;
+    for (uint32_t p = 0; p < cnt; p++) {
+        const uint32_t xl = P->xloc[p];
+        const uint64_t zm = P->z[p];
+        const int ph = (int)P->ph[p];
+        // Z parity of the thread part, then per register the register bits
+        const int kt = ph + 2 * __popcll(zm & gmem);
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const uint32_t t = tid | ((uint32_t)r << LT);
+            const V w = lds[swz<V>(t ^ xl)];  // psi(c ^ x); psi(c) when x = 0
+            int kr = 0;
+#pragma unroll
+            for (int j = 0; j < R; j++)
+                if ((r >> j) & 1) kr += (zm & rq[j]) ? 1 : 0;
+            const V tv = ipow_sel((kt + 2 * kr) & 3, v[r]);
+            const double tr = (double)tv.x, ti = (double)tv.y, xr = (double)w.x, xi = (double)w.y;
+            sr += xr * tr + xi * ti;  // conj(psi(c^x)) i^k psi(c)
+            si += xr * ti - xi * tr;
+        }
+        sr = wsum(sr);
+        si = wsum(si);
+        if (lane == 0) {
+            red[wv][2 * p] = sr;
+            red[wv][2 * p + 1] = si;
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * e.count) {
+        double s = 0.0;
+        for (int i = 0; i < NT / 64; i++) s += red[i][tid];
+        part[b * (2 * EXP_MAX) + tid] = s;
+    }
+}
+
+// sums nb x (2 EXP_MAX) partials into out[2 count], one block per value
+__global__ void k_expect_finish(const double *part, uint64_t nb, uint32_t count, double *out) {
+    __shared__ double red[256];
+    const uint32_t k = blockIdx.x;
+    double s = 0.0;
+    for (uint64_t i = threadIdx.x; i < nb; i += blockDim.x) s += part[i * (2 * EXP_MAX) + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+}
+
+template <typename T, int LT, int R>
+void expect_group_t(ncsv_state *s, const std::vector<uint32_t> &tq, const uint64_t *x, const uint64_t *z,
+                    const uint32_t *ph, const std::vector<uint32_t> &idx, double *out) {
+    using V = typename VecOf<T>::type;
+    constexpr int L = LT + R;
+    ExpArgs e;
+    memset(&e, 0, sizeof(e));
+    e.a.n = s->n;
+    uint64_t in = 0;
+    std::vector<int> pos_of(64, -1);
+    for (int p = 0; p < L; p++) {
+        e.a.tq[p] = (uint8_t)tq[p];
+        in |= 1ull << tq[p];
+        pos_of[tq[p]] = p;
+    }
+    int k = 0;
+    for (uint32_t q = 0; q < s->n; q++)
+        if (!((in >> q) & 1)) e.a.oq[k++] = (uint8_t)q;
+    e.count = (uint32_t)idx.size();
+    ExpParams P;
+    memset(&P, 0, sizeof(P));
+    for (size_t i = 0; i < idx.size(); i++) {
+        uint32_t xl = 0;
+        for (uint64_t a = x[idx[i]]; a; a &= a - 1) xl |= 1u << pos_of[__builtin_ctzll(a)];
+        P.xloc[i] = xl;
+        P.z[i] = z[idx[i]];
+        P.ph[i] = ph[idx[i]] & 3;
+    }
+    dev::ensure_tab(s, sizeof(ExpParams));
+    check_hip(hipMemcpyAsync(s->d_tab, &P, sizeof(P), hipMemcpyHostToDevice, s->stream), "upload Paulis");
+    const uint64_t tiles = 1ull << (s->n - L);
+    dev::ensure_part(s, tiles * 2 * EXP_MAX + 2 * EXP_MAX + 64);
+    double *fin = s->d_part + tiles * 2 * EXP_MAX;
+    {
+        KTimer kt(s, "expect_tile", (double)s->amp_bytes());
+        hipLaunchKernelGGL((k_expect_tile<T, LT, R>), dim3((uint32_t)tiles), dim3(1 << LT), 0, s->stream,
+                           (const V *)s->amp, e, (const ExpParams *)s->d_tab, s->d_part);
+        check_hip(hipGetLastError(), "k_expect_tile");
+    }
+    hipLaunchKernelGGL(k_expect_finish, dim3(2 * e.count), dim3(256), 0, s->stream, (const double *)s->d_part, tiles,
+                       e.count, fin);
+    check_hip(hipGetLastError(), "k_expect_finish");
+    check_hip(hipMemcpyAsync(s->h_part, fin, 2 * e.count * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+              "readback");
+    dev::sync(s);
+    for (size_t i = 0; i < idx.size(); i++) {
+        out[2 * idx[i]] = s->h_part[2 * i];
+        out[2 * idx[i] + 1] = s->h_part[2 * i + 1];
+    }
+}
+
+}  // namespace
+
+namespace dev {
+
+// Groups the Paulis so each group's X qubits fit in one tile (first fit),
+// then one read of the state per group.  Returns false (nothing done) when
+// the state is smaller than a tile or some Pauli's X part does not fit.
+bool expectations_tiled(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
+                        double *out) {
+    const TileGeom geo = tile_geom(s->prec);
+    const uint32_t L = geo.L(), c = geo.c;
+    if (s->n < L || s->n > 40) return false;
+    const uint64_t low = (1ull << c) - 1;
+    struct Group {
+        uint64_t T;
+        std::vector<uint32_t> idx;
+    };
+    std::vector<Group> groups;
+    for (uint32_t i = 0; i < count; i++) {
+        const uint64_t need = x[i] & ~low;
+        if ((uint32_t)__builtin_popcountll(need) > L - c) return false;
+        bool placed = false;
+        for (auto &g : groups) {
+            if (g.idx.size() < (size_t)EXP_MAX && (uint32_t)__builtin_popcountll(g.T | need) <= L) {
+                g.T |= need;
+                g.idx.push_back(i);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) groups.push_back({low | need, {i}});
+    }
+    for (auto &g : groups) {
+        // fill the tile with the lowest unused qubits
+        for (uint32_t q = 0; q < s->n && (uint32_t)__builtin_popcountll(g.T) < L; q++) g.T |= 1ull << q;
+        std::vector<uint32_t> tq;
+        for (uint32_t q = 0; q < s->n; q++)
+            if ((g.T >> q) & 1) tq.push_back(q);
+        if (s->prec == NCSV_C128)
+            expect_group_t<double, 8, 4>(s, tq, x, z, ph, g.idx, out);
+        else
+            expect_group_t<float, 8, 5>(s, tq, x, z, ph, g.idx, out);
+    }
+    return true;
 }
 
 }  // namespace dev
