@@ -205,7 +205,10 @@ ncsv_status ncsv_run(ncsv_state *s, const ncsv_circuit *c, ncsv_uniform_fn rng, 
 /* ---- execution options and counters --------------------------------------- */
 
 /* Gate fusion for ncsv_run: 0 = one device sweep per gate (the reference's
- * schedule), 1 = fused tile sweeps (default). */
+ * schedule); 1 = fused tile sweeps (default), run by kernels specialized per
+ * sweep (compiled at run time, cached) from 20 qubits on (NCSV_JIT_MIN_QUBITS)
+ * and by the op-list interpreter kernel below that; 2 = fused, always
+ * specialized.  NCSV_JIT=0 disables specialization. */
 ncsv_status ncsv_set_fusion(ncsv_state *s, int mode);
 /* Device-kernel timing with HIP events on the state's stream. */
 typedef struct {
@@ -229,6 +232,10 @@ ncsv_status ncsv_synchronize(const ncsv_state *s);
  * ops' semantics; stats[0..3] = sweeps, stages, ops, standalone gates.
  * Lets the planner be checked without a device. */
 ncsv_status ncsv_plan_emulate(const ncsv_circuit *c, ncsv_precision prec, double *amp, uint64_t *stats);
+/* Diagnostics (no counterpart in the reference): generates the specialized
+ * kernel of every tile sweep of a unitary circuit's plan and compiles it for
+ * `arch` (e.g. "gfx950") without a device; *nsweeps = kernels compiled. */
+ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const char *arch, uint32_t *nsweeps);
 
 #ifdef __cplusplus
 }

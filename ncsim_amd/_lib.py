@@ -69,6 +69,7 @@ _SIGS = {
     "ncsv_sweep_count": [ctypes.c_void_p, c_u64p],
     "ncsv_synchronize": [ctypes.c_void_p],
     "ncsv_plan_emulate": [ctypes.c_void_p, ctypes.c_int, c_dp, c_u64p],
+    "ncsv_jit_check": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, c_u32p],
 }
 
 _lib = None

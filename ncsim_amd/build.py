@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libncsv.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["lower.cpp", "circuit.cpp", "api.cpp", "fuse.cpp", "kernels.hip", "tile.hip"]
+SOURCES = ["lower.cpp", "circuit.cpp", "api.cpp", "fuse.cpp", "jit.cpp", "kernels.hip", "tile.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NCSV_ARCH", "gfx950")
@@ -27,7 +27,7 @@ DEVICE_ONLY = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src + ".o")
-    cmd = [HIPCC, "-c", path, "-o", obj] + COMMON
+    cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE_ONLY
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip", "-c", path, "-o", obj] + COMMON + DEVICE_ONLY
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -42,7 +42,7 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lhiprtc", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

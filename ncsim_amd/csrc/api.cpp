@@ -726,7 +726,7 @@ ncsv_status ncsv_run(ncsv_state *s, const ncsv_circuit *cc, ncsv_uniform_fn rng,
 ncsv_status ncsv_set_fusion(ncsv_state *s, int mode) {
     return guard([&] {
         need(s, "state");
-        if (mode < 0 || mode > 1) fail(NCSV_INVALID, "fusion mode must be 0 or 1");
+        if (mode < 0 || mode > 2) fail(NCSV_INVALID, "fusion mode must be 0, 1 or 2");
         s->fusion = mode;
     });
 }

@@ -9,6 +9,10 @@
 // of L tile qubits, splits the gates into stages.
 #include "fuse.hpp"
 
+#include <cstdlib>
+
+#include "jit.hpp"
+
 #include <algorithm>
 #include <cstring>
 
@@ -447,15 +451,27 @@ std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t
     return out;
 }
 
-// A cached plan: the segment's items and their device programs.
+// A cached plan: the segment's items and their device programs (the
+// interpreter's op lists, or the specialized kernels' coefficients).
 struct Plan {
     std::vector<SegItem> items;
     std::vector<size_t> prog_off;  // offset of each tile item's program in dbuf
+    std::vector<void *> kern;      // specialized kernel per tile item (JIT plans)
     void *dbuf = nullptr;
     ~Plan() {
+        for (void *k : kern)
+            if (k) jit::release(k);
         if (dbuf) (void)hipFree(dbuf);
     }
 };
+
+bool use_jit(const ncsv_state *s) {
+    if (!jit::available() || s->fusion == 0) return false;
+    if (s->fusion == 2) return true;
+    int minq = 20;
+    if (const char *e = getenv("NCSV_JIT_MIN_QUBITS")) minq = atoi(e);
+    return (int)s->n >= minq;
+}
 
 }  // namespace
 
@@ -501,7 +517,8 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         }
         return;
     }
-    const int64_t key = ((int64_t)i << 20) | ((int64_t)geo.key() << 2) | (int64_t)s->prec;
+    const bool jt = use_jit(s);
+    const int64_t key = ((int64_t)i << 21) | ((int64_t)geo.key() << 3) | ((int64_t)jt << 2) | (int64_t)s->prec;
     std::shared_ptr<Plan> plan;
     {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -517,14 +534,38 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         }
         plan->items = plan_segment(ops, s->n, geo);
         size_t total = 0;
-        for (auto &it : plan->items) {
-            plan->prog_off.push_back(total);
-            if (it.tile) total += dev::tile_prog_bytes(it.sw, s->prec);
-        }
-        if (total) {
-            std::vector<char> host(total);
+        std::vector<char> host;
+        plan->kern.assign(plan->items.size(), nullptr);
+        if (jt) {
+            // specialized kernels: sources compile in parallel; coefficients as T
+            const size_t tsz = s->prec == NCSV_C128 ? 8 : 4;
+            for (size_t k = 0; k < plan->items.size(); k++) {
+                plan->prog_off.push_back(total);
+                if (!plan->items[k].tile) continue;
+                std::vector<double> coef;
+                const std::string src = jit::generate(plan->items[k].sw, s->n, geo, s->prec == NCSV_C128, &coef);
+                plan->kern[k] = jit::prepare(src);
+                host.resize(total + (coef.size() * tsz + 255) / 256 * 256 + 256);
+                for (size_t t = 0; t < coef.size(); t++) {
+                    if (tsz == 8)
+                        memcpy(host.data() + total + 8 * t, &coef[t], 8);
+                    else {
+                        const float f = (float)coef[t];
+                        memcpy(host.data() + total + 4 * t, &f, 4);
+                    }
+                }
+                total = host.size();
+            }
+        } else {
+            for (auto &it : plan->items) {
+                plan->prog_off.push_back(total);
+                if (it.tile) total += dev::tile_prog_bytes(it.sw, s->prec);
+            }
+            host.resize(total);
             for (size_t k = 0; k < plan->items.size(); k++)
                 if (plan->items[k].tile) dev::tile_prog_pack(plan->items[k].sw, s->prec, host.data() + plan->prog_off[k]);
+        }
+        if (total) {
             check_hip(hipMalloc(&plan->dbuf, total), "hipMalloc(tile programs)");
             check_hip(hipMemcpy(plan->dbuf, host.data(), total, hipMemcpyHostToDevice), "upload tile programs");
         }
@@ -533,7 +574,9 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
     }
     for (size_t k = 0; k < plan->items.size(); k++) {
         const SegItem &it = plan->items[k];
-        if (it.tile)
+        if (it.tile && plan->kern[k])
+            jit::launch(s, plan->kern[k], geo.L(), geo.LT, (const char *)plan->dbuf + plan->prog_off[k]);
+        else if (it.tile)
             dev::run_tile_sweep(s, it.sw, (const char *)plan->dbuf + plan->prog_off[k], geo);
         else
             dev::apply_lop(s, it.op);

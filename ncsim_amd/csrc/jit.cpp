@@ -1,0 +1,501 @@
+// Sweep kernels specialized per plan: the stage/op list of one tile sweep
+// written out as straight-line HIP source and compiled at run time with
+// hiprtc for the device's architecture.  Register bits, masks, tile and
+// outer qubit positions and the LDS layouts of every stage become
+// constants; the gate coefficients stay data (read from a device array with
+// constant offsets).  X-type moves without conditions (X, CX-free swaps,
+// SWAP) cost nothing: the generator renames registers.  The compiled
+// kernels are cached by source text for the life of the process.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstring>
+#include <future>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "jit.hpp"
+
+namespace ncsv {
+namespace jit {
+
+namespace {
+
+struct Gen {
+    std::ostringstream o;
+    std::vector<double> *coef;
+    std::vector<int> name;  // logical register r -> variable index
+    uint32_t LT, R, L, NR;
+    std::vector<uint32_t> tq;
+    std::vector<uint32_t> oq;
+
+    std::string var(uint32_t r) const { return "a" + std::to_string(name[r]); }
+
+    uint32_t push(double v) {
+        coef->push_back(v);
+        return (uint32_t)coef->size() - 1;
+    }
+    std::string C(uint32_t i) const { return "c[" + std::to_string(i) + "]"; }
+};
+
+// Expression depositing bits 0..k-1 of `src` on the bit positions `dst`
+// (ascending source bits), grouping runs of consecutive destinations.
+std::string deposit(const std::string &src, const std::vector<uint32_t> &dst, bool wide) {
+    std::ostringstream e;
+    bool first = true;
+    size_t i = 0;
+    while (i < dst.size()) {
+        size_t j = i + 1;
+        while (j < dst.size() && dst[j] == dst[j - 1] + 1) j++;
+        const uint32_t len = (uint32_t)(j - i);
+        const uint64_t mask = (len >= 64) ? ~0ull : ((1ull << len) - 1);
+        if (!first) e << " | ";
+        first = false;
+        e << "(((" << src << " >> " << i << ") & " << mask << (wide ? "ull" : "u") << ") << " << dst[i] << ")";
+        i = j;
+    }
+    if (first) e << (wide ? "0ull" : "0u");
+    return e.str();
+}
+
+std::vector<uint32_t> positions(uint32_t mask, uint32_t L, bool set) {
+    std::vector<uint32_t> v;
+    for (uint32_t p = 0; p < L; p++)
+        if ((((mask >> p) & 1) != 0) == set) v.push_back(p);
+    return v;
+}
+
+// i^k * (variable) as an expression pair
+void ipow_assign(std::ostringstream &o, const std::string &dst, const std::string &src, int k) {
+    switch (k & 3) {
+    case 0: o << dst << " = " << src << ";"; break;
+    case 1: o << dst << ".x = -" << src << ".y; " << dst << ".y = " << src << ".x;"; break;
+    case 2: o << dst << ".x = -" << src << ".x; " << dst << ".y = -" << src << ".y;"; break;
+    default: o << dst << ".x = " << src << ".y; " << dst << ".y = -" << src << ".x;"; break;
+    }
+}
+
+// Condition on the global index bits outside the registers, in the
+// current layout: thread bits come from tid, outer bits from the base.
+std::string cond_expr(const Gen &g, uint64_t gmask, uint32_t cur) {
+    if (!gmask) return "";
+    const std::vector<uint32_t> tp = positions(cur, g.L, false);
+    uint64_t tmask = 0, bmask = 0;
+    for (uint64_t a = gmask; a; a &= a - 1) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        int pos = -1;
+        for (uint32_t p = 0; p < g.L; p++)
+            if (g.tq[p] == q) pos = (int)p;
+        if (pos < 0) {
+            bmask |= 1ull << q;
+        } else {
+            for (size_t i = 0; i < tp.size(); i++)
+                if (tp[i] == (uint32_t)pos) tmask |= 1ull << i;
+        }
+    }
+    std::ostringstream e;
+    e << "(";
+    bool first = true;
+    if (tmask) {
+        e << "((tid & " << tmask << "u) == " << tmask << "u)";
+        first = false;
+    }
+    if (bmask) {
+        if (!first) e << " && ";
+        e << "((base & " << bmask << "ull) == " << bmask << "ull)";
+    }
+    e << ")";
+    return e.str();
+}
+
+void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
+    auto &o = g.o;
+    const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
+    const std::vector<uint32_t> ra = positions(from, g.L, true), rb = positions(to, g.L, true);
+    o << "  {\n    const unsigned ta = " << deposit("tid", ta, false) << ";\n";
+    for (uint32_t r = 0; r < g.NR; r++) {
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if ((r >> j) & 1) off |= 1u << ra[j];
+        o << "    lds[swz(ta | " << off << "u)] = " << g.var(r) << ";\n";
+    }
+    o << "    __syncthreads();\n    const unsigned tb = " << deposit("tid", tb, false) << ";\n";
+    for (uint32_t r = 0; r < g.NR; r++) {
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if ((r >> j) & 1) off |= 1u << rb[j];
+        o << "    " << g.var(r) << " = lds[swz(tb | " << off << "u)];\n";
+    }
+    o << "    __syncthreads();\n  }\n";
+}
+
+bool is_exact(cd c, double re, double im) { return c.real() == re && c.imag() == im; }
+
+void emit_op(Gen &g, const TileOpH &op, uint32_t cur) {
+    auto &o = g.o;
+    const uint32_t NR = g.NR;
+    switch (op.kind) {
+    case TK_U1:
+    case TK_U1R:
+    case TK_U1X: {
+        const uint32_t J = op.j0;
+        uint32_t k0;
+        if (op.kind == TK_U1) {
+            k0 = g.push(op.m[0].real());
+            for (int t = 0; t < 4; t++) {
+                if (t) g.push(op.m[t].real());
+                g.push(op.m[t].imag());
+            }
+        } else if (op.kind == TK_U1R) {
+            k0 = g.push(op.m[0].real());
+            g.push(op.m[1].real());
+            g.push(op.m[2].real());
+            g.push(op.m[3].real());
+        } else {
+            k0 = g.push(op.m[0].real());
+            g.push(op.m[1].imag());
+            g.push(op.m[2].imag());
+            g.push(op.m[3].real());
+        }
+        o << "  {\n";
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & (1u << J)) continue;
+            const std::string a = g.var(i), b = g.var(i | (1u << J));
+            o << "    { const V x = " << a << ", y = " << b << ";\n";
+            if (op.kind == TK_U1) {
+                // m = [a0 a1 | b0 b1 | c0 c1 | d0 d1] at k0..k0+7
+                auto c = [&](int t) { return g.C(k0 + t); };
+                o << "      " << a << ".x = " << c(0) << "*x.x - " << c(1) << "*x.y + " << c(2) << "*y.x - " << c(3)
+                  << "*y.y;\n";
+                o << "      " << a << ".y = " << c(0) << "*x.y + " << c(1) << "*x.x + " << c(2) << "*y.y + " << c(3)
+                  << "*y.x;\n";
+                o << "      " << b << ".x = " << c(4) << "*x.x - " << c(5) << "*x.y + " << c(6) << "*y.x - " << c(7)
+                  << "*y.y;\n";
+                o << "      " << b << ".y = " << c(4) << "*x.y + " << c(5) << "*x.x + " << c(6) << "*y.y + " << c(7)
+                  << "*y.x; }\n";
+            } else if (op.kind == TK_U1R) {
+                auto c = [&](int t) { return g.C(k0 + t); };
+                o << "      " << a << ".x = " << c(0) << "*x.x + " << c(1) << "*y.x; " << a << ".y = " << c(0)
+                  << "*x.y + " << c(1) << "*y.y;\n";
+                o << "      " << b << ".x = " << c(2) << "*x.x + " << c(3) << "*y.x; " << b << ".y = " << c(2)
+                  << "*x.y + " << c(3) << "*y.y; }\n";
+            } else {
+                auto c = [&](int t) { return g.C(k0 + t); };
+                o << "      " << a << ".x = " << c(0) << "*x.x - " << c(1) << "*y.y; " << a << ".y = " << c(0)
+                  << "*x.y + " << c(1) << "*y.x;\n";
+                o << "      " << b << ".x = " << c(3) << "*y.x - " << c(2) << "*x.y; " << b << ".y = " << c(3)
+                  << "*y.y + " << c(2) << "*x.x; }\n";
+            }
+        }
+        o << "  }\n";
+        break;
+    }
+    case TK_U2: {
+        const uint32_t A = op.j0, B = op.j1;
+        const uint32_t k0 = g.push(op.m[0].real());
+        g.push(op.m[0].imag());
+        for (int t = 1; t < 16; t++) {
+            g.push(op.m[t].real());
+            g.push(op.m[t].imag());
+        }
+        o << "  {\n";
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & ((1u << A) | (1u << B))) continue;
+            const uint32_t ix[4] = {i, i | (1u << A), i | (1u << B), i | (1u << A) | (1u << B)};
+            o << "    { const V x0 = " << g.var(ix[0]) << ", x1 = " << g.var(ix[1]) << ", x2 = " << g.var(ix[2])
+              << ", x3 = " << g.var(ix[3]) << ";\n";
+            for (int a = 0; a < 4; a++) {
+                std::ostringstream re, im;
+                for (int b = 0; b < 4; b++) {
+                    const std::string mr = g.C(k0 + 2 * (4 * a + b)), mi = g.C(k0 + 2 * (4 * a + b) + 1);
+                    re << (b ? " + " : "") << mr << "*x" << b << ".x - " << mi << "*x" << b << ".y";
+                    im << (b ? " + " : "") << mr << "*x" << b << ".y + " << mi << "*x" << b << ".x";
+                }
+                o << "      " << g.var(ix[a]) << ".x = " << re.str() << ";\n";
+                o << "      " << g.var(ix[a]) << ".y = " << im.str() << ";\n";
+            }
+            o << "    }\n";
+        }
+        o << "  }\n";
+        break;
+    }
+    case TK_DM: {
+        const std::string cond = cond_expr(g, op.gmask, cur);
+        const cd ph = op.m[0];
+        std::string pre = "  ";
+        if (!cond.empty()) {
+            o << "  if " << cond << " {\n";
+            pre = "    ";
+        }
+        uint32_t kr = 0;
+        const bool neg = is_exact(ph, -1.0, 0.0), pi = is_exact(ph, 0.0, 1.0), mi = is_exact(ph, 0.0, -1.0);
+        if (!neg && !pi && !mi) {
+            kr = g.push(ph.real());
+            g.push(ph.imag());
+        }
+        for (uint32_t i = 0; i < NR; i++) {
+            if ((i & op.rmask) != op.rmask) continue;
+            const std::string a = g.var(i);
+            if (neg)
+                o << pre << a << ".x = -" << a << ".x; " << a << ".y = -" << a << ".y;\n";
+            else if (pi || mi) {
+                o << pre << "{ const V x = " << a << "; ";
+                ipow_assign(o, a, "x", pi ? 1 : 3);
+                o << " }\n";
+            } else
+                o << pre << "{ const V x = " << a << "; " << a << ".x = " << g.C(kr) << "*x.x - " << g.C(kr + 1)
+                  << "*x.y; " << a << ".y = " << g.C(kr) << "*x.y + " << g.C(kr + 1) << "*x.x; }\n";
+        }
+        if (!cond.empty()) o << "  }\n";
+        break;
+    }
+    case TK_PERM1: {
+        const std::string cond = cond_expr(g, op.gmask, cur);
+        const uint32_t J = op.j0;
+        if (cond.empty() && op.k0 == 0 && op.k1 == 0) {
+            // a plain move: rename the registers
+            for (uint32_t i = 0; i < NR; i++) {
+                if (i & (1u << J)) continue;
+                if ((i & op.rmask) != op.rmask) continue;
+                std::swap(g.name[i], g.name[i | (1u << J)]);
+            }
+            break;
+        }
+        std::string pre = "  ";
+        if (!cond.empty()) {
+            o << "  if " << cond << " {\n";
+            pre = "    ";
+        }
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & (1u << J)) continue;
+            if ((i & op.rmask) != op.rmask) continue;
+            const std::string a = g.var(i), b = g.var(i | (1u << J));
+            o << pre << "{ const V x = " << a << ", y = " << b << "; ";
+            ipow_assign(o, a, "y", op.k1);
+            o << " ";
+            ipow_assign(o, b, "x", op.k0);
+            o << " }\n";
+        }
+        if (!cond.empty()) o << "  }\n";
+        break;
+    }
+    case TK_SWAPR:
+        for (uint32_t i = 0; i < NR; i++)
+            if ((i & (1u << op.j0)) && !(i & (1u << op.j1)))
+                std::swap(g.name[i], g.name[(i & ~(1u << op.j0)) | (1u << op.j1)]);
+        break;
+    default: fail(NCSV_INVALID, "jit: unknown op");
+    }
+}
+
+}  // namespace
+
+std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef) {
+    Gen g;
+    g.coef = coef;
+    g.LT = geo.LT;
+    g.R = geo.R;
+    g.L = geo.L();
+    g.NR = 1u << geo.R;
+    g.tq = sw.tq;
+    g.name.resize(g.NR);
+    for (uint32_t r = 0; r < g.NR; r++) g.name[r] = (int)r;
+    uint64_t in = 0;
+    for (uint32_t q : sw.tq) in |= 1ull << q;
+    for (uint32_t q = 0; q < n; q++)
+        if (!((in >> q) & 1)) g.oq.push_back(q);
+    const uint32_t memr = ((1u << g.R) - 1) << g.LT;
+    bool lds = false;
+    {
+        uint32_t cur = memr;
+        for (auto &S : sw.stages) {
+            if (S.rmask_pos != cur) lds = true;
+            cur = S.rmask_pos;
+        }
+        if (cur != memr) lds = true;
+    }
+    auto &o = g.o;
+    const int S = dbl ? 4 : 5;
+    o << "typedef " << (dbl ? "double" : "float") << " T;\ntypedef " << (dbl ? "double2" : "float2") << " V;\n";
+    o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
+      << ")) & " << ((1u << S) - 1) << "u); }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
+      << ") void ncsv_sweep(V *__restrict__ amp, const T *__restrict__ c) {\n";
+    if (lds) o << "  __shared__ V lds[" << (1u << g.L) << "];\n";
+    o << "  const unsigned tid = threadIdx.x;\n  const unsigned long long b = blockIdx.x;\n";
+    o << "  const unsigned long long base = " << deposit("b", g.oq, true) << ";\n";
+    std::vector<uint32_t> low(sw.tq.begin(), sw.tq.begin() + g.LT);
+    // the thread bits go to the qubits of positions 0..LT-1 (not necessarily consecutive)
+    {
+        std::ostringstream e;
+        e << "base";
+        for (uint32_t i = 0; i < g.LT; i++) e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low[i] << ")";
+        o << "  const unsigned long long gm = " << e.str() << ";\n";
+    }
+    auto roff = [&](uint32_t r) {
+        uint64_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if ((r >> j) & 1) off |= 1ull << sw.tq[g.LT + j];
+        return off;
+    };
+    for (uint32_t r = 0; r < g.NR; r++) o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
+    uint32_t cur = memr;
+    for (auto &St : sw.stages) {
+        if (St.rmask_pos != cur) {
+            emit_exchange(g, cur, St.rmask_pos);
+            cur = St.rmask_pos;
+        }
+        for (uint32_t k = St.op_begin; k < St.op_end; k++) emit_op(g, sw.ops[k], cur);
+    }
+    if (cur != memr) emit_exchange(g, cur, memr);
+    for (uint32_t r = 0; r < g.NR; r++) o << "  amp[gm | " << roff(r) << "ull] = " << g.var(r) << ";\n";
+    o << "}\n";
+    return o.str();
+}
+
+// ------------------------------------------------------------------ compile
+
+namespace {
+
+struct Entry {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    std::string err;
+};
+
+std::mutex g_mu;
+std::map<std::string, std::shared_future<std::shared_ptr<Entry>>> g_cache;
+
+std::string arch_name() {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        std::string a = p.gcnArchName;
+        const size_t c = a.find(':');
+        return c == std::string::npos ? a : a.substr(0, c);
+    }
+    return "gfx950";
+}
+
+}  // namespace
+
+std::vector<char> compile_to_code(const std::string &src, const std::string &arch, std::string *log) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "ncsv_sweep.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        *log = "hiprtcCreateProgram failed";
+        return {};
+    }
+    const std::string a = "--offload-arch=" + arch;
+    const char *opts[] = {a.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    if (ls > 1) {
+        std::string l(ls, '\0');
+        hiprtcGetProgramLog(prog, &l[0]);
+        *log = l;
+    }
+    std::vector<char> code;
+    if (r == HIPRTC_SUCCESS) {
+        size_t cs = 0;
+        hiprtcGetCodeSize(prog, &cs);
+        code.resize(cs);
+        hiprtcGetCode(prog, code.data());
+    }
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(src);
+    if (it != g_cache.end()) return it->second;
+    const std::string arch = arch_name();
+    std::shared_future<std::shared_ptr<Entry>> f = std::async(std::launch::async, [src, arch]() {
+                                                         auto e = std::make_shared<Entry>();
+                                                         std::string log;
+                                                         std::vector<char> code = compile_to_code(src, arch, &log);
+                                                         if (code.empty()) {
+                                                             e->err = "sweep kernel compile failed: " + log;
+                                                             return e;
+                                                         }
+                                                         if (hipModuleLoadData(&e->mod, code.data()) != hipSuccess ||
+                                                             hipModuleGetFunction(&e->fn, e->mod, "ncsv_sweep") !=
+                                                                 hipSuccess) {
+                                                             e->err = "sweep kernel load failed";
+                                                             e->fn = nullptr;
+                                                         }
+                                                         return e;
+                                                     }).share();
+    g_cache[src] = f;
+    return f;
+}
+
+void *prepare(const std::string &src) {
+    auto f = get_or_start(src);
+    return (void *)new std::shared_future<std::shared_ptr<Entry>>(f);
+}
+
+void release(void *handle) { delete (std::shared_future<std::shared_ptr<Entry>> *)handle; }
+
+void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef) {
+    auto &f = *(std::shared_future<std::shared_ptr<Entry>> *)handle;
+    const std::shared_ptr<Entry> e = f.get();
+    if (!e->fn) fail(NCSV_DEVICE, e->err);
+    void *amp = s->amp;
+    const void *c = dcoef;
+    void *args[] = {&amp, &c};
+    const uint64_t tiles = 1ull << (s->n - L);
+    KTimer kt(s, "sweep_jit", 2.0 * (double)s->amp_bytes());
+    check_hip(hipModuleLaunchKernel(e->fn, (uint32_t)tiles, 1, 1, 1u << LT, 1, 1, 0, s->stream, args, nullptr),
+              "hipModuleLaunchKernel(sweep)");
+    s->sweeps++;
+}
+
+bool available() {
+    static int ok = -1;
+    if (ok < 0) {
+        const char *e = getenv("NCSV_JIT");
+        ok = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return ok == 1;
+}
+
+}  // namespace jit
+}  // namespace ncsv
+
+extern "C" {
+
+// Diagnostics: the specialized source of every tile sweep of a unitary
+// circuit's plan, compiled for `arch` (e.g. "gfx950") without a device.
+// Returns NCSV_OK when all compile; *nsweeps receives the count.
+ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const char *arch, uint32_t *nsweeps) {
+    try {
+        using namespace ncsv;
+        const TileGeom geo = dev::tile_geom(prec);
+        std::vector<LOp> ops;
+        for (const COp &op : c->ops) {
+            if (op.kind == CKind::Gate)
+                ops.push_back(lower_gate(op.g, op.qubits.data(), (uint32_t)op.qubits.size(), c->n));
+            else if (op.kind != CKind::Tick)
+                fail(NCSV_INVALID, "jit check takes unitary circuits");
+        }
+        const std::vector<SegItem> items = plan_segment(ops, c->n, geo);
+        uint32_t k = 0;
+        for (const SegItem &it : items) {
+            if (!it.tile) continue;
+            std::vector<double> coef;
+            const std::string src = jit::generate(it.sw, c->n, geo, prec == NCSV_C128, &coef);
+            std::string log;
+            if (jit::compile_to_code(src, arch ? arch : "gfx950", &log).empty()) fail(NCSV_DEVICE, "compile: " + log);
+            k++;
+        }
+        if (nsweeps) *nsweeps = k;
+        return NCSV_OK;
+    } catch (ncsv::Error &e) {
+        return e.code;
+    }
+}
+
+}  // extern "C"

@@ -345,3 +345,35 @@ def test_complex64_form_tracks_complex128():
     assert 1 - fid < 1e-5
     assert np.abs(va - vb).max() < 1e-5
     assert abs(a.norm_sq() - b.norm_sq()) < 1e-6
+
+
+def _mixed():
+    from tests.test_plan import MIXED
+
+    return MIXED + MIXED.split("\n", 1)[1] * 2
+
+
+@pytest.mark.parametrize("fusion", [1, 2])
+@pytest.mark.parametrize("name,text", [
+    ("mixed", None),
+    ("brickwork", W.rotation_brickwork(15, 12, 3)),
+    ("layers", W.random_layers(14, 20, 5, measure=False)),
+    ("qft", W.qft_with_rz(14, 40, 1)),
+])
+def test_fused_sweeps_match_the_oracle(name, text, fusion):
+    """The fused tile sweeps — op-list interpreter (fusion 1 below 20 qubits)
+    and specialized kernels (fusion 2) — against the oracle, every op kind."""
+    text = text or _mixed()
+    n = OCircuit.parse(text).n
+    a = rand_state(n, 21)
+    o = StateVector(n, a.copy())
+    o.run(OCircuit.parse(text), SplitMix64(0))
+    d = H.HipStateVector.from_amplitudes(n, a)
+    d.set_fusion(fusion)
+    d.run(H.Circuit.parse(text))
+    assert maxerr(o, d) < TOL
+    # complex64 form through the same sweeps
+    f = H.HipStateVector.from_amplitudes(n, a, H.C64)
+    f.set_fusion(fusion)
+    f.run(H.Circuit.parse(text))
+    assert np.abs(f.to_vector() - o.to_vector()).max() < 2e-5

@@ -90,3 +90,14 @@ def test_fusion_cuts_sweeps():
     assert st[0] <= 20, st
     st64 = emulate(text, H.C64)
     assert st64[0] <= st[0], (st, st64)
+
+
+@pytest.mark.parametrize("prec", [H.C128, H.C64])
+@pytest.mark.parametrize("name,text", [("mixed", MIXED), ("brickwork", W.rotation_brickwork(16, 6, 3))])
+def test_specialized_sweep_kernels_compile(name, text, prec):
+    """Every tile sweep of the plan, written out as a specialized kernel,
+    compiles for gfx950 (hiprtc, no device needed)."""
+    c = H.Circuit.parse(text)
+    k = ctypes.c_uint32()
+    H._check(L.lib().ncsv_jit_check(c._h, prec, b"gfx950", ctypes.byref(k)))
+    assert k.value >= 1
