@@ -236,6 +236,12 @@ ncsv_status ncsv_plan_emulate(const ncsv_circuit *c, ncsv_precision prec, double
  * kernel of every tile sweep of a unitary circuit's plan and compiles it for
  * `arch` (e.g. "gfx950") without a device; *nsweeps = kernels compiled. */
 ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const char *arch, uint32_t *nsweeps);
+/* Diagnostics (no counterpart in the reference): the specialized
+ * expectation kernels for `count` Paulis on n qubits, compiled for `arch`
+ * without a device; *ngroups = kernels; Unsupported when some X part does
+ * not fit a tile's register stage. */
+ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t count, const uint64_t *x,
+                                  const uint64_t *z, const uint32_t *ph, const char *arch, uint32_t *ngroups);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,8 @@ _SIGS = {
     "ncsv_synchronize": [ctypes.c_void_p],
     "ncsv_plan_emulate": [ctypes.c_void_p, ctypes.c_int, c_dp, c_u64p],
     "ncsv_jit_check": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, c_u32p],
+    "ncsv_jit_check_expect": [ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, c_u64p, c_u64p, c_u32p,
+                              ctypes.c_char_p, c_u32p],
 }
 
 _lib = None

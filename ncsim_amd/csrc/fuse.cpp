@@ -465,15 +465,17 @@ struct Plan {
     }
 };
 
-bool use_jit(const ncsv_state *s) {
+
+
+}  // namespace
+
+bool use_specialized(const ncsv_state *s) {
     if (!jit::available() || s->fusion == 0) return false;
     if (s->fusion == 2) return true;
     int minq = 20;
     if (const char *e = getenv("NCSV_JIT_MIN_QUBITS")) minq = atoi(e);
     return (int)s->n >= minq;
 }
-
-}  // namespace
 
 std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const TileGeom &geo) {
     std::vector<SegItem> items;
@@ -517,7 +519,7 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         }
         return;
     }
-    const bool jt = use_jit(s);
+    const bool jt = use_specialized(s);
     const int64_t key = ((int64_t)i << 21) | ((int64_t)geo.key() << 3) | ((int64_t)jt << 2) | (int64_t)s->prec;
     std::shared_ptr<Plan> plan;
     {

@@ -65,6 +65,9 @@ struct TileGeom {
 // The fused schedule of a segment of lowered gates on n qubits.
 std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const TileGeom &geo);
 
+// Whether the state runs kernels specialized per plan (fusion 2, or fusion 1
+// from NCSV_JIT_MIN_QUBITS = 20 qubits on; never with NCSV_JIT=0).
+bool use_specialized(const ncsv_state *s);
 // Applies the gate ops c->ops[i..j) (ticks and annotations skipped).
 void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j);
 // Frees the fused-engine buffers a state holds.
@@ -76,6 +79,8 @@ TileGeom tile_geom(ncsv_precision prec);
 // Bytes of a sweep's device program (stages, then ops) and its packing.
 size_t tile_prog_bytes(const SweepH &sw, ncsv_precision prec);
 void tile_prog_pack(const SweepH &sw, ncsv_precision prec, void *dst);
+// Sums each of `rows` contiguous rows of `cols` doubles; results to host.
+void rowsum(ncsv_state *s, const double *part, uint32_t rows, uint32_t cols, double *host_out);
 // Runs one tile sweep whose program sits at `dprog` in device memory.
 void run_tile_sweep(ncsv_state *s, const SweepH &sw, const void *dprog, const TileGeom &geo);
 }  // namespace dev

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstring>
 #include <future>
 #include <map>
@@ -354,6 +355,141 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     return o.str();
 }
 
+// ------------------------------------------------------- expectation kernels
+
+// Group kernel for Paulis whose X part lies on register positions of one of
+// the group's stages.  For each Pauli, per register pair (r, r ^ xr):
+// P = conj(psi(c^x)) psi(c); term(c) = (-1)^pt i^(ph + 2 pr(r)) P and
+// term(c^x) = (-1)^pt i^(ph + 2 pr(r^xr)) conj(P), pt the Z parity of the
+// thread and outer bits, pr that of the register bits.  One workgroup walks
+// tiles with a grid stride; partial sums go to part[k * gridDim.x + block].
+std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo, bool dbl) {
+    Gen g;
+    std::vector<double> dummy;
+    g.coef = &dummy;
+    g.LT = geo.LT;
+    g.R = geo.R;
+    g.L = geo.L();
+    g.NR = 1u << geo.R;
+    g.tq = eg.tq;
+    g.name.resize(g.NR);
+    for (uint32_t r = 0; r < g.NR; r++) g.name[r] = (int)r;
+    uint64_t in = 0;
+    for (uint32_t q : eg.tq) in |= 1ull << q;
+    for (uint32_t q = 0; q < n; q++)
+        if (!((in >> q) & 1)) g.oq.push_back(q);
+    std::vector<int> pos_of(64, -1);
+    for (uint32_t p = 0; p < g.L; p++) pos_of[eg.tq[p]] = (int)p;
+    const uint32_t memr = ((1u << g.R) - 1) << g.LT;
+    const uint32_t cnt = (uint32_t)eg.paulis.size();
+    bool lds = false;
+    {
+        uint32_t cur = memr;
+        for (auto &st : eg.stages) {
+            if (st.rpos != cur) lds = true;
+            cur = st.rpos;
+        }
+    }
+    auto &o = g.o;
+    const int S = dbl ? 4 : 5;
+    o << "typedef " << (dbl ? "double" : "float") << " T;\ntypedef " << (dbl ? "double2" : "float2") << " V;\n";
+    o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
+      << ")) & " << ((1u << S) - 1) << "u); }\n";
+    o << "__device__ __forceinline__ double wsum(double v) { for (int k = 32; k > 0; k >>= 1) v += __shfl_down(v, k, 64); return v; }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
+      << ") void ncsv_expect(const V *__restrict__ amp, double *__restrict__ part, unsigned long long ntiles) {\n";
+    if (lds) o << "  __shared__ V lds[" << (1u << g.L) << "];\n";
+    o << "  __shared__ double red[" << ((1u << g.LT) / 64) << "][" << 2 * cnt << "];\n";
+    o << "  const unsigned tid = threadIdx.x;\n";
+    for (uint32_t k = 0; k < cnt; k++) o << "  double sr" << k << " = 0.0, si" << k << " = 0.0;\n";
+    o << "  for (unsigned long long b = blockIdx.x; b < ntiles; b += gridDim.x) {\n";
+    o << "  const unsigned long long base = " << deposit("b", g.oq, true) << ";\n";
+    std::vector<uint32_t> low(eg.tq.begin(), eg.tq.begin() + g.LT);
+    {
+        std::ostringstream e;
+        e << "base";
+        for (uint32_t i = 0; i < g.LT; i++) e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low[i] << ")";
+        o << "  const unsigned long long gm = " << e.str() << ";\n";
+    }
+    auto roff = [&](uint32_t r) {
+        uint64_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if ((r >> j) & 1) off |= 1ull << eg.tq[g.LT + j];
+        return off;
+    };
+    for (uint32_t r = 0; r < g.NR; r++) o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
+    uint32_t cur = memr;
+    for (auto &st : eg.stages) {
+        if (st.rpos != cur) {
+            emit_exchange(g, cur, st.rpos);
+            cur = st.rpos;
+        }
+        // global bits of the thread in this layout: base | thread positions
+        const std::vector<uint32_t> tp = positions(cur, g.L, false), rp = positions(cur, g.L, true);
+        {
+            std::ostringstream e;
+            e << "base";
+            for (size_t i = 0; i < tp.size(); i++)
+                e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << eg.tq[tp[i]] << ")";
+            o << "  {\n  const unsigned long long g = " << e.str() << ";\n";
+        }
+        for (uint32_t k : st.paulis) {
+            const ExpPauli &P = eg.paulis[k];
+            // register masks of x and z; z on the rest
+            uint32_t xr = 0, zr = 0;
+            uint64_t zrest = P.z;
+            for (uint32_t j = 0; j < g.R; j++) {
+                const uint32_t q = eg.tq[rp[j]];
+                if ((P.x >> q) & 1) xr |= 1u << j;
+                if ((P.z >> q) & 1) {
+                    zr |= 1u << j;
+                    zrest &= ~(1ull << q);
+                }
+            }
+            o << "  { const bool odd = __popcll(g & " << zrest << "ull) & 1; double tr = 0.0, ti = 0.0;\n";
+            auto add = [&](int kk, const std::string &re, const std::string &im) {
+                // (tr, ti) += i^kk (re + i im)
+                switch (kk & 3) {
+                case 0: o << "    tr += " << re << "; ti += " << im << ";\n"; break;
+                case 1: o << "    tr -= " << im << "; ti += " << re << ";\n"; break;
+                case 2: o << "    tr -= " << re << "; ti -= " << im << ";\n"; break;
+                default: o << "    tr += " << im << "; ti -= " << re << ";\n"; break;
+                }
+            };
+            for (uint32_t r = 0; r < g.NR; r++) {
+                const uint32_t r2 = r ^ xr;
+                if (xr && r2 < r) continue;
+                const int kr = (int)(P.ph + 2 * __builtin_popcount(zr & r)) & 3;
+                const std::string w = g.var(r), u = g.var(r2);
+                if (!xr) {
+                    o << "    { const T x = " << w << ".x, y = " << w << ".y; const double m = (double)x*x + (double)y*y;\n";
+                    add(kr, "m", "0.0");
+                    o << "    }\n";
+                } else {
+                    const int kr2 = (int)(P.ph + 2 * __builtin_popcount(zr & r2)) & 3;
+                    o << "    { const double wx = " << w << ".x, wy = " << w << ".y, ux = " << u << ".x, uy = " << u
+                      << ".y;\n      const double pr = ux*wx + uy*wy, pi = ux*wy - uy*wx;\n";
+                    add(kr, "pr", "pi");
+                    add(kr2, "pr", "(-pi)");
+                    o << "    }\n";
+                }
+            }
+            o << "    if (odd) { tr = -tr; ti = -ti; }\n    sr" << k << " += tr; si" << k << " += ti; }\n";
+        }
+        o << "  }\n";
+    }
+    if (lds) o << "  __syncthreads();\n";
+    o << "  }\n";
+    o << "  const int lane = tid & 63, wv = tid >> 6;\n";
+    for (uint32_t k = 0; k < cnt; k++) {
+        o << "  { const double a = wsum(sr" << k << "), b2 = wsum(si" << k << "); if (lane == 0) { red[wv][" << 2 * k
+          << "] = a; red[wv][" << 2 * k + 1 << "] = b2; } }\n";
+    }
+    o << "  __syncthreads();\n  if (tid < " << 2 * cnt << ") { double s = 0.0; for (int i = 0; i < " << ((1u << g.LT) / 64)
+      << "; i++) s += red[i][tid]; part[(unsigned long long)tid * gridDim.x + blockIdx.x] = s; }\n}\n";
+    return o.str();
+}
+
 // ------------------------------------------------------------------ compile
 
 namespace {
@@ -407,12 +543,13 @@ std::vector<char> compile_to_code(const std::string &src, const std::string &arc
     return code;
 }
 
-std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src) {
+std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src, const char *fname = "ncsv_sweep") {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_cache.find(src);
     if (it != g_cache.end()) return it->second;
     const std::string arch = arch_name();
-    std::shared_future<std::shared_ptr<Entry>> f = std::async(std::launch::async, [src, arch]() {
+    const std::string fn = fname;
+    std::shared_future<std::shared_ptr<Entry>> f = std::async(std::launch::async, [src, arch, fn]() {
                                                          auto e = std::make_shared<Entry>();
                                                          std::string log;
                                                          std::vector<char> code = compile_to_code(src, arch, &log);
@@ -421,7 +558,7 @@ std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src) 
                                                              return e;
                                                          }
                                                          if (hipModuleLoadData(&e->mod, code.data()) != hipSuccess ||
-                                                             hipModuleGetFunction(&e->fn, e->mod, "ncsv_sweep") !=
+                                                             hipModuleGetFunction(&e->fn, e->mod, fn.c_str()) !=
                                                                  hipSuccess) {
                                                              e->err = "sweep kernel load failed";
                                                              e->fn = nullptr;
@@ -434,6 +571,11 @@ std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src) 
 
 void *prepare(const std::string &src) {
     auto f = get_or_start(src);
+    return (void *)new std::shared_future<std::shared_ptr<Entry>>(f);
+}
+
+void *prepare_named(const std::string &src, const char *fname) {
+    auto f = get_or_start(src, fname);
     return (void *)new std::shared_future<std::shared_ptr<Entry>>(f);
 }
 
@@ -453,6 +595,20 @@ void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dc
     s->sweeps++;
 }
 
+void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_t grid, double *dpart) {
+    auto &f = *(std::shared_future<std::shared_ptr<Entry>> *)handle;
+    const std::shared_ptr<Entry> e = f.get();
+    if (!e->fn) fail(NCSV_DEVICE, e->err);
+    const void *amp = s->amp;
+    unsigned long long ntiles = 1ull << (s->n - L);
+    void *args[] = {&amp, &dpart, &ntiles};
+    KTimer kt(s, "expect_jit", (double)s->amp_bytes());
+    check_hip(hipModuleLaunchKernel(e->fn, grid, 1, 1, 1u << LT, 1, 1, 0, s->stream, args, nullptr),
+              "hipModuleLaunchKernel(expect)");
+}
+
+void *prepare_named(const std::string &src, const char *fname);
+
 bool available() {
     static int ok = -1;
     if (ok < 0) {
@@ -462,10 +618,157 @@ bool available() {
     return ok == 1;
 }
 
+// Expectation values through specialized group kernels.  Returns false
+// (nothing done) when some Pauli's X part does not fit one tile's register
+// stage.
+bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
+                 const uint32_t *ph, std::vector<ExpGroup> *out_groups) {
+    const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, L), LT = geo.LT;
+    if (n < L) return false;
+    ncsv_state fake;
+    fake.n = n;
+    ncsv_state *s = &fake;
+    const uint64_t low = (1ull << c) - 1;
+    constexpr uint32_t GMAX = 16;
+    for (uint32_t i = 0; i < count; i++)
+        if ((uint32_t)__builtin_popcountll(x[i]) > R || (uint32_t)__builtin_popcountll(x[i] & ~low) > L - c) return false;
+    // tiles: first fit on the union of the X qubits above the low ones
+    struct G0 {
+        uint64_t T;
+        std::vector<uint32_t> idx;
+    };
+    std::vector<G0> g0;
+    for (uint32_t i = 0; i < count; i++) {
+        const uint64_t need = x[i] & ~low;
+        bool placed = false;
+        for (auto &g : g0)
+            if (g.idx.size() < GMAX && (uint32_t)__builtin_popcountll(g.T | need) <= L) {
+                g.T |= need;
+                g.idx.push_back(i);
+                placed = true;
+                break;
+            }
+        if (!placed) g0.push_back({low | need, {i}});
+    }
+    std::vector<ExpGroup> groups;
+    for (auto &g : g0) {
+        ExpGroup eg;
+        eg.index = g.idx;
+        // register stages over qubits: union of X qubits <= R
+        std::vector<std::pair<uint64_t, std::vector<uint32_t>>> st;
+        for (uint32_t k = 0; k < g.idx.size(); k++) {
+            const uint64_t xq = x[g.idx[k]];
+            bool placed = false;
+            for (auto &sg : st)
+                if ((uint32_t)__builtin_popcountll(sg.first | xq) <= R) {
+                    sg.first |= xq;
+                    sg.second.push_back(k);
+                    placed = true;
+                    break;
+                }
+            if (!placed) st.push_back({xq, {k}});
+            eg.paulis.push_back({x[g.idx[k]], z[g.idx[k]], ph[g.idx[k]] & 3});
+        }
+        // tile qubits: the low ones, every X qubit, then the lowest others
+        uint64_t T = g.T;
+        for (auto &sg : st) T |= sg.first;
+        for (uint32_t q = 0; q < s->n && (uint32_t)__builtin_popcountll(T) < L; q++) T |= 1ull << q;
+        // positions: low qubits fixed; stage 0's other X qubits on the top
+        eg.tq.assign(L, ~0u);
+        std::vector<int> pos_of(64, -1);
+        for (uint32_t p = 0; p < c; p++) {
+            eg.tq[p] = p;
+            pos_of[p] = (int)p;
+        }
+        uint32_t top = L - R;
+        for (uint64_t a = st[0].first & ~low; a; a &= a - 1) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(a);
+            eg.tq[top] = q;
+            pos_of[q] = (int)top++;
+        }
+        uint32_t next = c;
+        for (uint64_t a = T; a; a &= a - 1) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(a);
+            if (pos_of[q] >= 0) continue;
+            while (eg.tq[next] != ~0u) next++;
+            eg.tq[next] = q;
+            pos_of[q] = (int)next;
+        }
+        uint32_t prev = ((1u << R) - 1) << LT;
+        for (auto &sg : st) {
+            uint32_t rpos = 0;
+            for (uint64_t a = sg.first; a; a &= a - 1) rpos |= 1u << pos_of[__builtin_ctzll(a)];
+            for (uint32_t p = L; p-- > 0 && (uint32_t)__builtin_popcount(rpos) < R;)
+                if ((prev >> p) & 1) rpos |= 1u << p;
+            for (uint32_t p = L; p-- > 0 && (uint32_t)__builtin_popcount(rpos) < R;) rpos |= 1u << p;
+            eg.stages.push_back({rpos, sg.second});
+            prev = rpos;
+        }
+        groups.push_back(std::move(eg));
+    }
+    *out_groups = std::move(groups);
+    return true;
+}
+
+bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
+                  double *out) {
+    const TileGeom geo = dev::tile_geom(s->prec);
+    const uint32_t L = geo.L(), LT = geo.LT;
+    constexpr uint32_t GMAX = 16;
+    std::vector<ExpGroup> groups;
+    if (!plan_expect(s->n, geo, count, x, z, ph, &groups)) return false;
+    const bool dbl = s->prec == NCSV_C128;
+    std::vector<void *> kern;
+    for (auto &eg : groups) kern.push_back(prepare_named(generate_expect(eg, s->n, geo, dbl), "ncsv_expect"));
+    const uint64_t tiles = 1ull << (s->n - L);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, 2048);
+    dev::ensure_part(s, (size_t)2 * GMAX * grid + 2 * GMAX + 64);
+    std::vector<double> res(2 * GMAX);
+    try {
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            const uint32_t cnt = (uint32_t)groups[gi].paulis.size();
+            launch_expect(s, kern[gi], L, LT, grid, s->d_part);
+            dev::rowsum(s, s->d_part, 2 * cnt, grid, res.data());
+            for (uint32_t k = 0; k < cnt; k++) {
+                out[2 * groups[gi].index[k]] = res[2 * k];
+                out[2 * groups[gi].index[k] + 1] = res[2 * k + 1];
+            }
+        }
+    } catch (...) {
+        for (void *k : kern) release(k);
+        throw;
+    }
+    for (void *k : kern) release(k);
+    return true;
+}
+
 }  // namespace jit
 }  // namespace ncsv
 
 extern "C" {
+
+// Diagnostics: the specialized expectation kernels for `count` Paulis on n
+// qubits, compiled for `arch` without a device; *ngroups = kernels.
+ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t count, const uint64_t *x,
+                                  const uint64_t *z, const uint32_t *ph, const char *arch, uint32_t *ngroups) {
+    try {
+        using namespace ncsv;
+        const TileGeom geo = dev::tile_geom(prec);
+        std::vector<jit::ExpGroup> groups;
+        if (!jit::plan_expect(n, geo, count, x, z, ph, &groups)) return NCSV_UNSUPPORTED;
+        for (auto &eg : groups) {
+            std::string log;
+            if (jit::compile_to_code(jit::generate_expect(eg, n, geo, prec == NCSV_C128), arch ? arch : "gfx950", &log)
+                    .empty())
+                fail(NCSV_DEVICE, "compile: " + log);
+        }
+        if (ngroups) *ngroups = (uint32_t)groups.size();
+        return NCSV_OK;
+    } catch (ncsv::Error &e) {
+        return e.code;
+    }
+}
+
 
 // Diagnostics: the specialized source of every tile sweep of a unitary
 // circuit's plan, compiled for `arch` (e.g. "gfx950") without a device.

@@ -19,5 +19,33 @@ void release(void *handle);
 void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef);
 // false when NCSV_JIT=0.
 bool available();
+
+// A group of Paulis measured by one specialized kernel over one tile shape:
+// the tile's qubits (positions 0..L-1) and stages of register positions,
+// each with the Paulis whose X part lies on its registers.
+struct ExpPauli {
+    uint64_t x, z;
+    uint32_t ph;
+};
+struct ExpStage {
+    uint32_t rpos;
+    std::vector<uint32_t> paulis;
+};
+struct ExpGroup {
+    std::vector<uint32_t> tq;
+    std::vector<ExpPauli> paulis;
+    std::vector<ExpStage> stages;
+    std::vector<uint32_t> index;  // position of each Pauli in the caller's list
+};
+std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo, bool dbl);
+void *prepare_named(const std::string &src, const char *fname);
+// part: 2 x paulis rows of `grid` partial sums
+void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_t grid, double *dpart);
+// <P_k> for all k through specialized group kernels; false when some X part
+// does not fit a tile's register stage (nothing done then).
+bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
+                 const uint32_t *ph, std::vector<ExpGroup> *out_groups);
+bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
+                  double *out);
 }  // namespace jit
 }  // namespace ncsv

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <vector>
 
+#include "jit.hpp"
 #include "ncsv_internal.hpp"
 
 namespace ncsv {
@@ -860,6 +861,7 @@ void expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64
             xm[i] = x[i] & mask;
             zm[i] = z[i] & mask;
         }
+        if (count > 1 && use_specialized(s) && jit::expectations(s, count, xm.data(), zm.data(), ph, out)) return;
         if (count > 1 && expectations_tiled(s, count, xm.data(), zm.data(), ph, out)) return;
     }
     for (uint32_t p0 = 0; p0 < count; p0 += MAXP) {

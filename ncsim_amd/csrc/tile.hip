@@ -750,7 +750,38 @@ void expect_group_t(ncsv_state *s, const std::vector<uint32_t> &tq, const uint64
     }
 }
 
+
+__global__ void k_rowsum(const double *part, uint32_t cols, double *out) {
+    __shared__ double red[256];
+    const uint64_t row = blockIdx.x;
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < cols; i += blockDim.x) s += part[row * cols + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[row] = red[0];
+}
+
 }  // namespace
+
+namespace dev {
+
+void rowsum(ncsv_state *s, const double *part, uint32_t rows, uint32_t cols, double *host_out) {
+    if (rows > 64) fail(NCSV_INVALID, "rowsum: too many rows");
+    double *fin = nullptr;
+    check_hip(hipMallocAsync((void **)&fin, rows * sizeof(double), s->stream), "hipMallocAsync(rowsum)");
+    hipLaunchKernelGGL(k_rowsum, dim3(rows), dim3(256), 0, s->stream, part, cols, fin);
+    check_hip(hipGetLastError(), "k_rowsum");
+    check_hip(hipMemcpyAsync(s->h_part, fin, rows * sizeof(double), hipMemcpyDeviceToHost, s->stream), "readback");
+    check_hip(hipFreeAsync(fin, s->stream), "hipFreeAsync");
+    sync(s);
+    memcpy(host_out, s->h_part, rows * sizeof(double));
+}
+
+}  // namespace dev
 
 namespace dev {
 

@@ -118,10 +118,14 @@ def test_pauli_expectation_and_projection():
     assert np.array_equal(w1.to_vector(), w2.to_vector())
 
 
-def test_batched_expectations_match_single():
+@pytest.mark.parametrize("fusion", [1, 2])
+def test_batched_expectations_match_single(fusion):
+    """Grouped expectations (LDS-partner kernel below 20 qubits, specialized
+    kernels with fusion 2) against the oracle's one-at-a-time values."""
     n = 16
     o, d = pair(n, 4)
-    ps = [Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 21, 5)]
+    d.set_fusion(fusion)
+    ps = [Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 40, 5, max_weight=6)]
     got = d.expectations(ps)
     for p, g in zip(ps, got):
         assert abs(o.expectation(p) - g) < TOL

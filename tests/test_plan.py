@@ -101,3 +101,16 @@ def test_specialized_sweep_kernels_compile(name, text, prec):
     k = ctypes.c_uint32()
     H._check(L.lib().ncsv_jit_check(c._h, prec, b"gfx950", ctypes.byref(k)))
     assert k.value >= 1
+
+
+@pytest.mark.parametrize("prec", [H.C128, H.C64])
+def test_specialized_expectation_kernels_compile(prec):
+    n = 30
+    ps = W.random_paulis(n, 64, 4)
+    k = len(ps)
+    xs = (ctypes.c_uint64 * k)(*[p[0] for p in ps])
+    zs = (ctypes.c_uint64 * k)(*[p[1] for p in ps])
+    ph = (ctypes.c_uint32 * k)(*[p[2] for p in ps])
+    g = ctypes.c_uint32()
+    H._check(L.lib().ncsv_jit_check_expect(n, prec, k, xs, zs, ph, b"gfx950", ctypes.byref(g)))
+    assert 1 <= g.value <= 12
