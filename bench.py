@@ -55,6 +55,64 @@ def cpu_baseline(seconds=15.0):
                       f"{dt:.1f} s, {threads} OpenMP threads"}
 
 
+def run_qft(args):
+    """Precision form: QFT (H + CPHASE(1, 2..n)) with 2000 RZ, a seeded X
+    prefix.  complex128 and complex64 at n-1 qubits (fidelity and max-abs
+    error of complex64 against complex128, on the device), then complex64 at
+    n qubits.  value = complex64 amp-updates/s at n qubits."""
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+
+    n = args.qubits if args.qubits != 30 else 33
+    res = {}
+
+    def timed(m, prec):
+        text = W.qft_with_rz(m, 2000, 7)
+        gates = W.count_gates(text)
+        circ = H.Circuit.parse(text)
+        s = H.HipStateVector.zero_state(m, prec)
+        s.set_fusion(args.fusion)
+        s.run(circ)  # warm-up (plans, kernels)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.set_basis(0)
+            s.run(circ)
+        s.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / args.steps
+        s.set_basis(0)
+        s.profile_reset()
+        s.profile(True)
+        s.run(circ)
+        s.profile(False)
+        st = s.profile_read()
+        log(f"qft: n={m} prec={prec} gates={gates} {ms:.1f} ms  " +
+            ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(st.items())))
+        return s, ms, gates, st
+
+    a, ms128, g1, st128 = timed(n - 1, H.C128)
+    b, ms64s, _, _ = timed(n - 1, H.C64)
+    cmp = a.compare(b)
+    del a, b
+    c, ms64, g2, st64 = timed(n, H.C64)
+    del c
+    name, (launches, tot_ms, byts) = max(st64.items(), key=lambda kv: kv[1][1])
+    achieved = byts / launches / (tot_ms / launches / 1000.0)
+    out = {"metric": "amp-updates/s", "value": g2 * float(1 << n) / (ms64 / 1000.0), "unit": "amp-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms64, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32, f64 reductions)",
+           "data": "synthetic (seeded circuit)",
+           "config": {"workload": f"QFT + 2000 RZ, {n} qubits complex64; {n - 1} qubits complex128 vs complex64",
+                      "gates": g2},
+           "precision_check": {"qubits": n - 1, "fidelity": cmp["fidelity"], "fidelity_loss": 1 - cmp["fidelity"],
+                               "max_abs": cmp["max_abs"], "ms_c128": ms128, "ms_c64": ms64s,
+                               "speedup_c64_over_c128": ms128 / ms64s},
+           "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None, "launches": launches,
+                        "avg_ms": tot_ms / launches}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,11 +122,16 @@ def main():
     ap.add_argument("--depth", type=int, default=24)
     ap.add_argument("--fusion", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "qft"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the state-vector path runs on a single "
                                    "MI355X; sharding across GPUs is described in DESIGN.md but not built"}))
         sys.exit(2)
+
+    if args.workload == "qft":
+        run_qft(args)
+        return
 
     import ncsim_amd as H
     from ncsim_amd import workloads as W

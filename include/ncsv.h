@@ -141,6 +141,10 @@ ncsv_status ncsv_expectation_sum(const ncsv_state *s, uint32_t count, const uint
                                  const uint32_t *phase, const double *coeffs, double out[2]);
 /* QuantumState::inner (vector.rs:431-443): <a|b>. */
 ncsv_status ncsv_inner(const ncsv_state *a, const ncsv_state *b, double out[2]);
+/* No counterpart in the reference (for checking the complex64 form): two
+ * states of the same size and any precisions, in one read of each:
+ * out = {Re<a|b>, Im<a|b>, <a|a>, <b|b>, max_i |a_i - b_i|} (FP64). */
+ncsv_status ncsv_compare(const ncsv_state *a, const ncsv_state *b, double out[5]);
 /* QuantumState::probability (mod.rs:104-108). */
 ncsv_status ncsv_probability(const ncsv_state *s, uint64_t index, double *out);
 

@@ -327,6 +327,18 @@ class HipStateVector:
         _check(L.lib().ncsv_inner(self._h, other._h, out))
         return complex(out[0], out[1])
 
+    def compare(self, other):
+        """(fidelity, max-abs difference, <self|other>, norms) against a
+        state of any precision, computed on the device in FP64."""
+        out = (ctypes.c_double * 5)()
+        _check(L.lib().ncsv_compare(self._h, other._h, out))
+        ip = complex(out[0], out[1])
+        fid = abs(ip) ** 2 / (out[2] * out[3]) if out[2] > 0 and out[3] > 0 else 0.0
+        return {"fidelity": fid, "max_abs": out[4], "inner": ip, "norm_self": out[2], "norm_other": out[3]}
+
+    def set_basis(self, x):
+        _check(L.lib().ncsv_set_basis(self._h, int(x)))
+
     def probability(self, x):
         out = ctypes.c_double()
         _check(L.lib().ncsv_probability(self._h, int(x), ctypes.byref(out)))

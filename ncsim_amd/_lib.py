@@ -40,6 +40,7 @@ _SIGS = {
     "ncsv_expectation_batch": [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u64p, c_u32p, c_dp],
     "ncsv_expectation_sum": [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u64p, c_u32p, c_dp, c_dp],
     "ncsv_inner": [ctypes.c_void_p, ctypes.c_void_p, c_dp],
+    "ncsv_compare": [ctypes.c_void_p, ctypes.c_void_p, c_dp],
     "ncsv_probability": [ctypes.c_void_p, ctypes.c_uint64, c_dp],
     "ncsv_project": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, c_dp],
     "ncsv_measure": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, UNIFORM_FN,

@@ -432,6 +432,25 @@ ncsv_status ncsv_inner(const ncsv_state *a, const ncsv_state *b, double out[2]) 
     });
 }
 
+ncsv_status ncsv_compare(const ncsv_state *a, const ncsv_state *b, double out[5]) {
+    return guard([&] {
+        need(a, "state");
+        need(b, "other");
+        need(out, "out");
+        std::unique_lock<std::recursive_mutex> l1(a->mu, std::defer_lock), l2(b->mu, std::defer_lock);
+        if (a == b)
+            l1.lock();
+        else if (a < b) {
+            l1.lock();
+            l2.lock();
+        } else {
+            l2.lock();
+            l1.lock();
+        }
+        dev::compare(const_cast<ncsv_state *>(a), const_cast<ncsv_state *>(b), out);
+    });
+}
+
 ncsv_status ncsv_probability(const ncsv_state *s, uint64_t index, double *out) {
     return guard([&] {
         need(s, "state");

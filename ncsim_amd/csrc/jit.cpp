@@ -10,6 +10,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <future>
 #include <map>
@@ -290,6 +291,93 @@ void emit_op(Gen &g, const TileOpH &op, uint32_t cur) {
     }
 }
 
+// A run of consecutive diagonal ops (phase c where the register bits hold
+// rmask and the other index bits hold gmask) as one phase per amplitude:
+// per register mask m, P_m = (host product of the unconditional phases) x
+// (thread product of the conditional ones), and every amplitude r is
+// multiplied by the P_m with m inside r.  Products by 1, -1, i, -i stay
+// exact.
+void emit_diag_run(Gen &g, const std::vector<const TileOpH *> &run, uint32_t cur) {
+    auto &o = g.o;
+    std::map<uint32_t, cd> uncond;
+    std::map<std::pair<uint32_t, uint64_t>, cd> cond;
+    for (const TileOpH *op : run) {
+        if (op->gmask == 0) {
+            auto it = uncond.find(op->rmask);
+            uncond[op->rmask] = it == uncond.end() ? op->m[0] : it->second * op->m[0];
+        } else {
+            const auto key = std::make_pair(op->rmask, op->gmask);
+            auto it = cond.find(key);
+            cond[key] = it == cond.end() ? op->m[0] : it->second * op->m[0];
+        }
+    }
+    std::vector<uint32_t> masks;
+    for (auto &kv : uncond)
+        if (kv.second != cd(1.0, 0.0)) masks.push_back(kv.first);
+    for (auto &kv : cond)
+        if (std::find(masks.begin(), masks.end(), kv.first.first) == masks.end()) masks.push_back(kv.first.first);
+    if (masks.empty()) return;
+    o << "  { // diagonal run of " << run.size() << " ops\n";
+    auto cm = [&](const std::string &p, const std::string &re, const std::string &im) {
+        o << "    { const T pr = " << p << ".x, pi = " << p << ".y; " << p << ".x = pr*" << re << " - pi*" << im << "; "
+          << p << ".y = pr*" << im << " + pi*" << re << "; }\n";
+    };
+    for (uint32_t m : masks) {
+        auto it = uncond.find(m);
+        const cd u = it == uncond.end() ? cd(1.0, 0.0) : it->second;
+        if (u == cd(1.0, 0.0)) {
+            o << "    V P" << m << "; P" << m << ".x = 1; P" << m << ".y = 0;\n";
+        } else {
+            const uint32_t k = g.push(u.real());
+            g.push(u.imag());
+            o << "    V P" << m << "; P" << m << ".x = " << g.C(k) << "; P" << m << ".y = " << g.C(k + 1) << ";\n";
+        }
+    }
+    for (auto &kv : cond) {
+        if (kv.second == cd(1.0, 0.0)) continue;
+        const uint32_t m = kv.first.first;
+        const std::string ce = cond_expr(g, kv.first.second, cur);
+        const uint32_t k = g.push(kv.second.real());
+        g.push(kv.second.imag());
+        o << "    if " << ce << " {\n  ";
+        cm("P" + std::to_string(m), g.C(k), g.C(k + 1));
+        o << "    }\n";
+    }
+    for (uint32_t r = 0; r < g.NR; r++) {
+        for (uint32_t m : masks) {
+            if ((r & m) != m) continue;
+            const std::string a = g.var(r);
+            o << "    { const V x = " << a << "; " << a << ".x = P" << m << ".x*x.x - P" << m << ".y*x.y; " << a
+              << ".y = P" << m << ".x*x.y + P" << m << ".y*x.x; }\n";
+        }
+    }
+    o << "  }\n";
+}
+
+// Folding a diagonal run pays when it needs fewer complex products per thread
+// than applying its ops one by one (sign flips and quarter turns count a
+// third of a product).
+bool diag_run_pays(const std::vector<const TileOpH *> &run, uint32_t NR) {
+    auto nreg = [&](uint32_t m) {
+        uint32_t k = 0;
+        for (uint32_t r = 0; r < NR; r++) k += (r & m) == m;
+        return k;
+    };
+    double single = 0.0;
+    std::map<uint32_t, int> masks;
+    std::map<std::pair<uint32_t, uint64_t>, int> conds;
+    for (const TileOpH *op : run) {
+        const cd c = op->m[0];
+        const bool special = c == cd(-1, 0) || c == cd(0, 1) || c == cd(0, -1);
+        single += nreg(op->rmask) * (special ? 0.34 : 1.0);
+        masks[op->rmask] = 1;
+        if (op->gmask) conds[{op->rmask, op->gmask}] = 1;
+    }
+    double folded = (double)conds.size();
+    for (auto &kv : masks) folded += nreg(kv.first);
+    return folded < single;
+}
+
 }  // namespace
 
 std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef) {
@@ -347,7 +435,24 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
             emit_exchange(g, cur, St.rmask_pos);
             cur = St.rmask_pos;
         }
-        for (uint32_t k = St.op_begin; k < St.op_end; k++) emit_op(g, sw.ops[k], cur);
+        std::vector<const TileOpH *> run;
+        auto flush = [&]() {
+            if (run.size() > 1 && diag_run_pays(run, g.NR)) {
+                emit_diag_run(g, run, cur);
+            } else {
+                for (const TileOpH *op : run) emit_op(g, *op, cur);
+            }
+            run.clear();
+        };
+        for (uint32_t k = St.op_begin; k < St.op_end; k++) {
+            if (sw.ops[k].kind == TK_DM) {
+                run.push_back(&sw.ops[k]);
+                continue;
+            }
+            flush();
+            emit_op(g, sw.ops[k], cur);
+        }
+        flush();
     }
     if (cur != memr) emit_exchange(g, cur, memr);
     for (uint32_t r = 0; r < g.NR; r++) o << "  amp[gm | " << roff(r) << "ull] = " << g.var(r) << ";\n";
@@ -517,6 +622,14 @@ std::string arch_name() {
 }  // namespace
 
 std::vector<char> compile_to_code(const std::string &src, const std::string &arch, std::string *log) {
+    if (const char *dir = getenv("NCSV_JIT_DUMP")) {
+        // keep the generated source for inspection
+        const std::string path = std::string(dir) + "/ncsv_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
+        if (FILE *f = fopen(path.c_str(), "w")) {
+            fwrite(src.data(), 1, src.size(), f);
+            fclose(f);
+        }
+    }
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "ncsv_sweep.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
         *log = "hiprtcCreateProgram failed";

@@ -51,6 +51,7 @@ template <typename V> __device__ __forceinline__ V ipow_mul(int k, V a) {
 }
 
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
 
 // Insert zero bits at the ascending positions sq[0..k) of a fiber number.
 __device__ __forceinline__ uint64_t deposit_zeros(uint64_t t, const int *sq, int k) {
@@ -75,26 +76,26 @@ template <typename T> struct OpArgs {
 template <typename T, int K>
 __global__ __launch_bounds__(TPB) void k_dense_small(typename VecOf<T>::type *a, uint64_t nf, OpArgs<T> g) {
     using V = typename VecOf<T>::type;
-    const uint64_t t = gtid();
-    if (t >= nf) return;
     constexpr int D = 1 << K;
-    const uint64_t base = deposit_zeros(t, g.sq, K);
-    V v[D];
+    for (uint64_t t = gtid(); t < nf; t += gstride()) {
+        const uint64_t base = deposit_zeros(t, g.sq, K);
+        V v[D];
 #pragma unroll
-    for (int b = 0; b < D; b++) v[b] = a[base + g.off[b]];
+        for (int b = 0; b < D; b++) v[b] = a[base + g.off[b]];
 #pragma unroll
-    for (int r = 0; r < D; r++) {
-        V acc;
-        acc.x = 0;
-        acc.y = 0;
+        for (int r = 0; r < D; r++) {
+            V acc;
+            acc.x = 0;
+            acc.y = 0;
 #pragma unroll
-        for (int c = 0; c < D; c++) {
-            V m;
-            m.x = g.re[r * D + c];
-            m.y = g.im[r * D + c];
-            acc = caddv(acc, cmulv(m, v[c]));
+            for (int c = 0; c < D; c++) {
+                V m;
+                m.x = g.re[r * D + c];
+                m.y = g.im[r * D + c];
+                acc = caddv(acc, cmulv(m, v[c]));
+            }
+            a[base + g.off[r]] = acc;
         }
-        a[base + g.off[r]] = acc;
     }
 }
 
@@ -109,19 +110,19 @@ __global__ __launch_bounds__(TPB) void k_dense_mid(typename VecOf<T>::type *a, u
     for (int i = threadIdx.x; i < D * D; i += blockDim.x) sm[i] = mat[i];
     for (int i = threadIdx.x; i < D; i += blockDim.x) so[i] = offs[i];
     __syncthreads();
-    const uint64_t t = gtid();
-    if (t >= nf) return;
-    const uint64_t base = deposit_zeros(t, g.sq, K);
-    V v[D];
+    for (uint64_t t = gtid(); t < nf; t += gstride()) {
+        const uint64_t base = deposit_zeros(t, g.sq, K);
+        V v[D];
 #pragma unroll
-    for (int b = 0; b < D; b++) v[b] = a[base + so[b]];
-    for (int r = 0; r < D; r++) {
-        V acc;
-        acc.x = 0;
-        acc.y = 0;
+        for (int b = 0; b < D; b++) v[b] = a[base + so[b]];
+        for (int r = 0; r < D; r++) {
+            V acc;
+            acc.x = 0;
+            acc.y = 0;
 #pragma unroll
-        for (int c = 0; c < D; c++) acc = caddv(acc, cmulv(sm[r * D + c], v[c]));
-        a[base + so[r]] = acc;
+            for (int c = 0; c < D; c++) acc = caddv(acc, cmulv(sm[r * D + c], v[c]));
+            a[base + so[r]] = acc;
+        }
     }
 }
 
@@ -130,24 +131,24 @@ template <typename T>
 __global__ __launch_bounds__(TPB) void k_dense_big(const typename VecOf<T>::type *in, typename VecOf<T>::type *out,
                                                    uint64_t N, int k, const int *q, const typename VecOf<T>::type *mat) {
     using V = typename VecOf<T>::type;
-    const uint64_t i = gtid();
-    if (i >= N) return;
-    uint64_t base = i;
-    uint32_t row = 0;
-    for (int j = 0; j < k; j++) {
-        row |= (uint32_t)((i >> q[j]) & 1) << j;
-        base &= ~(1ull << q[j]);
+    for (uint64_t i = gtid(); i < N; i += gstride()) {
+        uint64_t base = i;
+        uint32_t row = 0;
+        for (int j = 0; j < k; j++) {
+            row |= (uint32_t)((i >> q[j]) & 1) << j;
+            base &= ~(1ull << q[j]);
+        }
+        const uint32_t D = 1u << k;
+        V acc;
+        acc.x = 0;
+        acc.y = 0;
+        for (uint32_t c = 0; c < D; c++) {
+            uint64_t idx = base;
+            for (int j = 0; j < k; j++) idx |= (uint64_t)((c >> j) & 1) << q[j];
+            acc = caddv(acc, cmulv(mat[(uint64_t)row * D + c], in[idx]));
+        }
+        out[i] = acc;
     }
-    const uint32_t D = 1u << k;
-    V acc;
-    acc.x = 0;
-    acc.y = 0;
-    for (uint32_t c = 0; c < D; c++) {
-        uint64_t idx = base;
-        for (int j = 0; j < k; j++) idx |= (uint64_t)((c >> j) & 1) << q[j];
-        acc = caddv(acc, cmulv(mat[(uint64_t)row * D + c], in[idx]));
-    }
-    out[i] = acc;
 }
 
 // ------------------------------------------------------------ diagonal
@@ -155,18 +156,18 @@ __global__ __launch_bounds__(TPB) void k_dense_big(const typename VecOf<T>::type
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_diag(typename VecOf<T>::type *a, uint64_t nf, OpArgs<T> g) {
     using V = typename VecOf<T>::type;
-    const uint64_t t = gtid();
-    if (t >= nf) return;
-    const uint64_t base = deposit_zeros(t, g.sq, g.k);
     const int D = 1 << g.k;
-    for (int x = 0; x < D; x++) {
-        if ((g.unit >> x) & 1) continue;
-        V c;
-        c.x = g.re[x];
-        c.y = g.im[x];
-        const uint64_t i = base + g.off[x];
-        const V v = a[i];
-        if (!czero(v)) a[i] = cmulv(c, v);
+    for (uint64_t t = gtid(); t < nf; t += gstride()) {
+        const uint64_t base = deposit_zeros(t, g.sq, g.k);
+        for (int x = 0; x < D; x++) {
+            if ((g.unit >> x) & 1) continue;
+            V c;
+            c.x = g.re[x];
+            c.y = g.im[x];
+            const uint64_t i = base + g.off[x];
+            const V v = a[i];
+            if (!czero(v)) a[i] = cmulv(c, v);
+        }
     }
 }
 
@@ -174,21 +175,21 @@ __global__ __launch_bounds__(TPB) void k_diag(typename VecOf<T>::type *a, uint64
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_perm(typename VecOf<T>::type *a, uint64_t nf, OpArgs<T> g) {
     using V = typename VecOf<T>::type;
-    const uint64_t t = gtid();
-    if (t >= nf) return;
-    const uint64_t base = deposit_zeros(t, g.sq, g.k);
     const int D = 1 << g.k;
-    V v[16];
-    for (int x = 0; x < D; x++) v[x] = a[base + g.off[x]];
-    for (int x = 0; x < D; x++) {
-        V w = v[x];
-        if (!((g.unit >> x) & 1)) {
-            V c;
-            c.x = g.re[x];
-            c.y = g.im[x];
-            w = cmulv(c, w);
+    for (uint64_t t = gtid(); t < nf; t += gstride()) {
+        const uint64_t base = deposit_zeros(t, g.sq, g.k);
+        V v[16];
+        for (int x = 0; x < D; x++) v[x] = a[base + g.off[x]];
+        for (int x = 0; x < D; x++) {
+            V w = v[x];
+            if (!((g.unit >> x) & 1)) {
+                V c;
+                c.x = g.re[x];
+                c.y = g.im[x];
+                w = cmulv(c, w);
+            }
+            a[base + g.off[g.img[x]]] = w;
         }
-        a[base + g.off[g.img[x]]] = w;
     }
 }
 
@@ -196,23 +197,23 @@ __global__ __launch_bounds__(TPB) void k_perm(typename VecOf<T>::type *a, uint64
 // (P psi)(c) = i^{ph + 2|z & (c^x)|} psi(c ^ x)  (vector.rs:143-164)
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_pauli_diag(typename VecOf<T>::type *a, uint64_t N, uint64_t z, int ph) {
-    const uint64_t c = gtid();
-    if (c >= N) return;
-    const int k = (ph + 2 * __popcll(z & c)) & 3;
-    if (k) a[c] = ipow_mul(k, a[c]);
+    for (uint64_t c = gtid(); c < N; c += gstride()) {
+        const int k = (ph + 2 * __popcll(z & c)) & 3;
+        if (k) a[c] = ipow_mul(k, a[c]);
+    }
 }
 
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_pauli_pair(typename VecOf<T>::type *a, uint64_t np, uint64_t x, uint64_t z,
                                                      int ph, int h) {
     using V = typename VecOf<T>::type;
-    const uint64_t t = gtid();
-    if (t >= np) return;
-    const uint64_t c = ((t >> h) << (h + 1)) | (t & ((1ull << h) - 1));
-    const uint64_t d = c ^ x;
-    const V va = a[c], vb = a[d];
-    a[c] = ipow_mul((ph + 2 * __popcll(z & d)) & 3, vb);
-    a[d] = ipow_mul((ph + 2 * __popcll(z & c)) & 3, va);
+    for (uint64_t t = gtid(); t < np; t += gstride()) {
+        const uint64_t c = ((t >> h) << (h + 1)) | (t & ((1ull << h) - 1));
+        const uint64_t d = c ^ x;
+        const V va = a[c], vb = a[d];
+        a[c] = ipow_mul((ph + 2 * __popcll(z & d)) & 3, vb);
+        a[d] = ipow_mul((ph + 2 * __popcll(z & c)) & 3, va);
+    }
 }
 
 // ------------------------------------------------------------ reductions
@@ -326,6 +327,47 @@ __global__ void k_finish(const double *part, int nblocks, int W, double *out) {
     }
 }
 
+// -------------------------------------------------------------- compare
+// Two states of any precisions, scaled by their FP64 factors: sum conj(a) b,
+// sum |a|^2, sum |b|^2 and max |a - b| in one read of each.
+template <typename VA, typename VB>
+__global__ __launch_bounds__(TPB) void k_compare(const VA *a, const VB *b, uint64_t N, double sa, double sb,
+                                                 double *part) {
+    Comp acc[4];
+#pragma unroll
+    for (int w = 0; w < 4; w++) acc[w].init();
+    double mx = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = gtid(); c < N; c += stride) {
+        const VA va = a[c];
+        const VB vb = b[c];
+        const double ar = (double)va.x * sa, ai = (double)va.y * sa, br = (double)vb.x * sb, bi = (double)vb.y * sb;
+        acc[0].add(ar * br + ai * bi);
+        acc[1].add(ar * bi - ai * br);
+        acc[2].add(ar * ar + ai * ai);
+        acc[3].add(br * br + bi * bi);
+        const double dr = ar - br, di = ai - bi;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+    double v[5] = {acc[0].val(), acc[1].val(), acc[2].val(), acc[3].val(), 0.0};
+    // block max of mx in slot 4, sums in 0..3
+    __shared__ double red[TPB / 64][5];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int w = 0; w < 4; w++) {
+        const double t = wave_sum(v[w]);
+        if (lane == 0) red[wv][w] = t;
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    if (lane == 0) red[wv][4] = mx;
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double t = threadIdx.x == 4 ? 0.0 : 0.0;
+        for (int i = 0; i < TPB / 64; i++)
+            t = threadIdx.x == 4 ? fmax(t, red[i][4]) : t + red[i][threadIdx.x];
+        part[(uint64_t)blockIdx.x * 5 + threadIdx.x] = t;
+    }
+}
+
 // ------------------------------------------------------------- collapse
 // Z-type projector: keep or zero each amplitude (exact), weight of the kept.
 template <typename T>
@@ -388,25 +430,25 @@ __global__ __launch_bounds__(TPB) void k_scale(typename VecOf<T>::type *a, uint6
     // products round separately, as in the reference's complex multiply
 #pragma clang fp contract(off)
     using V = typename VecOf<T>::type;
-    const uint64_t i = gtid();
-    if (i >= N) return;
-    const V v = a[i];
-    if (czero(v)) return;
-    V r;
-    r.x = v.x * cr - v.y * ci;
-    r.y = v.x * ci + v.y * cr;
-    a[i] = r;
+    for (uint64_t i = gtid(); i < N; i += gstride()) {
+        const V v = a[i];
+        if (czero(v)) continue;
+        V r;
+        r.x = v.x * cr - v.y * ci;
+        r.y = v.x * ci + v.y * cr;
+        a[i] = r;
+    }
 }
 
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_fill(typename VecOf<T>::type *a, uint64_t N, T re, uint64_t one_index) {
     using V = typename VecOf<T>::type;
-    const uint64_t i = gtid();
-    if (i >= N) return;
-    V v;
-    v.x = (one_index == ~0ull) ? re : (i == one_index ? (T)1 : (T)0);
-    v.y = 0;
-    a[i] = v;
+    for (uint64_t i = gtid(); i < N; i += gstride()) {
+        V v;
+        v.x = (one_index == ~0ull) ? re : (i == one_index ? (T)1 : (T)0);
+        v.y = 0;
+        a[i] = v;
+    }
 }
 
 template <typename T>
@@ -414,21 +456,19 @@ __global__ __launch_bounds__(TPB) void k_kron(const typename VecOf<T>::type *a, 
                                               const typename VecOf<T>::type *b, typename VecOf<T>::type *out,
                                               uint64_t N) {
     using V = typename VecOf<T>::type;
-    const uint64_t i = gtid();
-    if (i >= N) return;
-    const V va = a[i & ((1ull << na) - 1)], vb = b[i >> na];
-    V r;
-    r.x = 0;
-    r.y = 0;
-    if (!czero(va) && !czero(vb)) r = cmulv(va, vb);
-    out[i] = r;
+    for (uint64_t i = gtid(); i < N; i += gstride()) {
+        const V va = a[i & ((1ull << na) - 1)], vb = b[i >> na];
+        V r;
+        r.x = 0;
+        r.y = 0;
+        if (!czero(va) && !czero(vb)) r = cmulv(va, vb);
+        out[i] = r;
+    }
 }
 
 template <typename TI, typename TO>
 __global__ __launch_bounds__(TPB) void k_convert(const TI *in, TO *out, uint64_t n2, double s) {
-    const uint64_t i = gtid();
-    if (i >= n2) return;
-    out[i] = (TO)((double)in[i] * s);
+    for (uint64_t i = gtid(); i < n2; i += gstride()) out[i] = (TO)((double)in[i] * s);
 }
 
 // -------------------------------------------------------------- sampling
@@ -485,8 +525,7 @@ __global__ __launch_bounds__(TPB) void k_sample(const typename VecOf<T>::type *a
                                                 const double *total, const double *us, uint64_t shots,
                                                 uint64_t *out) {
     using V = typename VecOf<T>::type;
-    const uint64_t s = gtid();
-    if (s >= shots) return;
+    for (uint64_t s = gtid(); s < shots; s += gstride()) {
     const double u = us[s] * *total;
     uint64_t lo = 0, hi = nblk;  // first block with u < excl + w
     while (lo < hi) {
@@ -510,10 +549,16 @@ __global__ __launch_bounds__(TPB) void k_sample(const typename VecOf<T>::type *a
     uint64_t pick = idx;
     while (pick > start && czero(a[pick])) pick--;
     out[s] = pick;
+    }
 }
 
 // ---------------------------------------------------------------- helpers
-inline uint32_t nblocks(uint64_t items) { return (uint32_t)((items + TPB - 1) / TPB); }
+// Blocks for a grid-stride launch over `items`: at most 2^20 blocks, so
+// grid x block stays well inside 32 bits for any state size.
+inline uint32_t nblocks(uint64_t items) {
+    const uint64_t b = (items + TPB - 1) / TPB;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 20));
+}
 
 inline uint32_t red_blocks(uint64_t N) {
     uint64_t b = (N + TPB * 8 - 1) / (TPB * 8);
@@ -916,6 +961,36 @@ void inner(ncsv_state *a, ncsv_state *b, double out[2]) {
         reduce_t<double, R_INNER, 2>(a, b->amp, r, out, "inner", 2.0 * (double)a->amp_bytes());
     else
         reduce_t<float, R_INNER, 2>(a, b->amp, r, out, "inner", 2.0 * (double)a->amp_bytes());
+}
+
+template <typename VA, typename VB> static void compare_t(ncsv_state *a, ncsv_state *b, double out[5]) {
+    const uint64_t N = 1ull << a->n;
+    const uint32_t nb = red_blocks(N);
+    ensure_part(a, (size_t)nb * 5 + 64);
+    {
+        KTimer kt(a, "compare", (double)a->amp_bytes() + (double)b->amp_bytes());
+        hipLaunchKernelGGL((k_compare<VA, VB>), dim3(nb), dim3(TPB), 0, a->stream, (const VA *)a->amp,
+                           (const VB *)b->amp, N, a->lscale, b->lscale, a->d_part);
+        launch_check("k_compare");
+    }
+    std::vector<double> h((size_t)nb * 5);
+    check_hip(hipMemcpyAsync(h.data(), a->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream),
+              "readback");
+    sync(a);
+    for (int w = 0; w < 5; w++) out[w] = 0.0;
+    for (uint32_t i = 0; i < nb; i++) {
+        for (int w = 0; w < 4; w++) out[w] += h[(size_t)i * 5 + w];
+        out[4] = std::max(out[4], h[(size_t)i * 5 + 4]);
+    }
+}
+
+void compare(ncsv_state *a, ncsv_state *b, double out[5]) {
+    if (a->n != b->n) fail(NCSV_INVALID, "compare: register sizes differ");
+    sync(b);
+    if (a->prec == NCSV_C128 && b->prec == NCSV_C128) compare_t<double2, double2>(a, b, out);
+    else if (a->prec == NCSV_C128) compare_t<double2, float2>(a, b, out);
+    else if (b->prec == NCSV_C128) compare_t<float2, double2>(a, b, out);
+    else compare_t<float2, float2>(a, b, out);
 }
 
 template <typename T> static double project_t(ncsv_state *s, uint64_t x, uint64_t z, uint32_t ph, int outcome) {

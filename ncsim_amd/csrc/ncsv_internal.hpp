@@ -167,6 +167,8 @@ void expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64
 // <Z-type or general P> real part and norm together (measure).
 void expect_and_norm(ncsv_state *s, uint64_t x, uint64_t z, uint32_t ph, double *e, double *nn);
 void inner(ncsv_state *a, ncsv_state *b, double out[2]);
+// <a|b>, |a|^2, |b|^2 (FP64, logical scale) and max |a - b|, any precisions.
+void compare(ncsv_state *a, ncsv_state *b, double out[5]);
 // collapse + weight of the stored amplitudes in one sweep.
 double project(ncsv_state *s, uint64_t x, uint64_t z, uint32_t ph, int outcome);
 void scale(ncsv_state *s, cd c);

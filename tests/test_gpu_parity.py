@@ -381,3 +381,21 @@ def test_fused_sweeps_match_the_oracle(name, text, fusion):
     f.set_fusion(fusion)
     f.run(H.Circuit.parse(text))
     assert np.abs(f.to_vector() - o.to_vector()).max() < 2e-5
+
+
+def test_large_state_round_trip():
+    """32 qubits, complex64 (2^32 amplitudes: launches need grid-stride
+    loops): H on every qubit gate by gate gives the uniform state (closed
+    form 2^-16); the fused sweeps undo it back to |0...0>."""
+    n = 32
+    s = H.HipStateVector.zero_state(n, H.C64)
+    s.set_fusion(0)
+    text = f"QUBITS {n}\nH " + " ".join(map(str, range(n))) + "\n"
+    s.run(H.Circuit.parse(text))
+    for x in (0, 1, (1 << n) - 1, 0x9E3779B9):
+        assert abs(s.amplitude(x) - 2.0 ** -16) < 1e-6 * 2.0 ** -16
+    assert abs(s.norm_sq() - 1) < 1e-6
+    s.set_fusion(1)
+    s.run(H.Circuit.parse(text))
+    assert abs(s.amplitude(0) - 1) < 1e-5
+    assert abs(s.amplitude(12345)) < 1e-6
