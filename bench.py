@@ -113,6 +113,147 @@ def run_qft(args):
     print(json.dumps(out), flush=True)
 
 
+def run_batch(args):
+    """Batched form: `--circuits` uneven circuits (10-22 qubits, depth
+    10-120, H/S/T/SQRT_X/RZ layers, CX matchings, CCZ with p = 0.05) with 8
+    Paulis each, all in one persistent launch per run.  Compared with
+    per-circuit launches (each circuit on its own state through the fused
+    single-state path) over every `--stride`-th circuit, scaled to the batch.
+    value = circuits/s of the batched form."""
+    import numpy as np
+
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+
+    prec = H.C64 if args.precision == "c64" else H.C128
+    specs = W.batch_circuits(args.circuits, 11)
+    circs = [H.Circuit.parse(t) for _, t, _ in specs]
+    paulis = [[H.Pauli(n, x, z, ph) for x, z, ph in ps] for n, _, ps in specs]
+    gates = np.array([W.count_gates(t) for _, t, _ in specs], dtype=np.float64)
+    ns = np.array([n for n, _, _ in specs])
+    amp_updates = float((gates * np.exp2(ns)).sum())
+    b = H.HipBatch(prec)
+    if args.tile_bits:
+        b.set_tile_bits(args.tile_bits)
+    for c, ps in zip(circs, paulis):
+        b.add(c, ps)
+    t0 = time.perf_counter()
+    b.run()  # plans every circuit and uploads the programs
+    cold = time.perf_counter() - t0
+    log(f"batch: first run {cold:.2f} s (planning {b.stats()['plan_ms']:.0f} ms)")
+    for _ in range(max(0, args.warmup - 1)):
+        b.run()
+    t0 = time.perf_counter()
+    kms = 0.0
+    for _ in range(args.steps):
+        b.run()
+        kms += b.stats()["kernel_ms"]
+    wall = (time.perf_counter() - t0) / args.steps
+    kms /= args.steps
+    st = b.stats()
+    log(f"batch: {wall * 1000:.1f} ms per run (kernel {kms:.1f} ms), {st['steps']:.0f} steps, {st['tiles']:.0f} tiles")
+    # per-circuit launches over a sample
+    os.environ["NCSV_JIT_MIN_QUBITS"] = "99"  # no per-circuit kernel compiles
+    sample = list(range(0, len(specs), args.stride))
+    states = {}
+    for i in sample:  # warm-up: plans cached on each circuit
+        n = specs[i][0]
+        s = states.setdefault((n, prec), H.HipStateVector.zero_state(n, prec))
+        s.set_basis(0)
+        s.run(circs[i])
+        s.expectations(paulis[i])
+    for s in states.values():
+        s.synchronize()
+    t0 = time.perf_counter()
+    maxdiff = 0.0
+    for i in sample:
+        n = specs[i][0]
+        s = states[(n, prec)]
+        s.set_basis(0)
+        s.run(circs[i])
+        ev = s.expectations(paulis[i])
+        bv = b.result(i)[0]
+        maxdiff = max(maxdiff, max(abs(a - c) for a, c in zip(ev, bv)))
+    per = (time.perf_counter() - t0) / len(sample)
+    del states
+    log(f"per-circuit: {per * 1000:.2f} ms per circuit over {len(sample)} circuits; max |batch - single| = {maxdiff:.2e}")
+    cps = len(specs) / wall
+    out = {"metric": "circuits/s", "value": cps, "unit": "circuits/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1000, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c64 (f32, f64 reductions)" if prec == H.C64 else "c128 (f64)",
+           "data": "synthetic (seeded circuits)",
+           "config": {"workload": f"{len(specs)} circuits, 10-22 qubits, depth 10-120, 8 Paulis each",
+                      "gates": int(gates.sum()), "tile_bits": args.tile_bits or None},
+           "amp_updates_per_s": amp_updates / wall, "kernel_ms": kms, "first_run_s": cold,
+           "per_circuit": {"circuits_per_s": 1.0 / per, "sample": len(sample), "ms_per_circuit": per * 1000,
+                           "path": "single-state fused interpreter, one state per qubit count"},
+           "speedup_vs_per_circuit": cps * per, "max_abs_batch_vs_single": maxdiff}
+    print(json.dumps(out), flush=True)
+
+
+def run_traj(args):
+    """Noisy trajectories: 24-qubit depth-40 brickwork of Haar SU(2) and CX
+    with DEPOLARIZE1(1e-3) after every gate and M+R on 4 qubits every 10
+    layers; `--trajectories` complex64 trajectories (SplitMix64 seeds) in
+    the batched form; the first 256 again in complex128 with the same draws
+    and the complex64 run's measurement outcomes forced.  Estimators: <Z_i>
+    and <Z_i Z_i+1> per trajectory (normalized).  value = trajectories/s."""
+    import numpy as np
+
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+
+    n = 24 if args.qubits == 30 else args.qubits
+    text = W.noisy_brickwork(n, 40, 5)
+    circ = H.Circuit.parse(text)
+    ps = [H.Pauli(n, 0, 1 << i, 0) for i in range(n)] + [H.Pauli(n, 0, 3 << i, 0) for i in range(n - 1)]
+    T = args.trajectories
+
+    def make(prec, count):
+        b = H.HipBatch(prec)
+        for _ in range(count):
+            b.add(circ, ps)
+        return b
+
+    b64 = make(H.C64, T)
+    seeds = np.arange(T, dtype=np.uint64) + 1000
+    b64.run(seeds=seeds)  # warm-up (planning, programs)
+    t0 = time.perf_counter()
+    b64.run(seeds=seeds)
+    wall = time.perf_counter() - t0
+    st = b64.stats()
+    est64 = []
+    for i in range(T):
+        ev, nn, oc = b64.result(i)
+        est64.append([e.real / nn for e in ev])
+    est64 = np.array(est64)
+    log(f"trajectories: {T} x {n} qubits complex64 in {wall:.2f} s (kernel {st['kernel_ms']:.0f} ms, "
+        f"{st['launches']:.0f} launches)")
+    m = min(256, T)
+    b128 = make(H.C128, m)
+    forced = []
+    for i in range(m):
+        oc = b64.result(i)[2]
+        forced.append(np.array([0xff if o is None else o for o in oc], dtype=np.uint8))
+    t1 = time.perf_counter()
+    b128.run(seeds=seeds[:m], forced=forced)
+    w128 = time.perf_counter() - t1
+    est128 = np.array([[e.real / b128.result(i)[1] for e in b128.result(i)[0]] for i in range(m)])
+    diff = float(np.abs(est64[:m] - est128).max())
+    log(f"check: {m} complex128 trajectories in {w128:.2f} s, max |est64 - est128| = {diff:.2e}")
+    gates = W.count_gates(text)
+    out = {"metric": "trajectories/s", "value": T / wall, "unit": "trajectories/s", "n_gpus": 1, "steps": 1,
+           "warmup": 1, "ms_per_step": wall * 1000, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c64 (f32, f64 reductions)", "data": "synthetic (seeded circuit and draws)",
+           "config": {"workload": f"{T} noisy trajectories, {n} qubits, depth 40, DEPOLARIZE1(1e-3), M+R every 10 layers",
+                      "gates": gates},
+           "amp_updates_per_s": gates * float(1 << n) * T / wall,
+           "estimates": {"mean_Z": est64[:, :n].mean(axis=0).tolist()[:4],
+                         "mean_ZZ": est64[:, n:].mean(axis=0).tolist()[:4]},
+           "fp64_check": {"trajectories": m, "max_abs_estimate_diff": diff, "seconds_c128": w128}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,15 +263,26 @@ def main():
     ap.add_argument("--depth", type=int, default=24)
     ap.add_argument("--fusion", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "qft"])
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "qft", "batch", "trajectories"])
+    ap.add_argument("--circuits", type=int, default=4096)
+    ap.add_argument("--stride", type=int, default=8)
+    ap.add_argument("--trajectories", type=int, default=2000)
+    ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
+    ap.add_argument("--tile-bits", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the state-vector path runs on a single "
-                                   "MI355X; sharding across GPUs is described in DESIGN.md but not built"}))
+                                   "MI355X; sharding across GPUs is not built"}))
         sys.exit(2)
 
     if args.workload == "qft":
         run_qft(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
+        return
+    if args.workload == "trajectories":
+        run_traj(args)
         return
 
     import ncsim_amd as H

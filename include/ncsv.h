@@ -6,7 +6,7 @@
  * every declaration (src/state/mod.rs = trait and provided methods,
  * src/state/vector.rs = the dense implementation).  A Rust `-sys` crate
  * declares these functions verbatim; a safe `HipStateVector` wrapper maps
- * `Gate`/`Pauli`/`Circuit` onto them (see INTEGRATION.md).
+ * `Gate`/`Pauli`/`Circuit` onto them.
  *
  * Conventions
  *  - Basis index bit q = qubit q (vector.rs:14).  A gate's local index has
@@ -246,6 +246,61 @@ ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const cha
  * not fit a tile's register stage. */
 ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t count, const uint64_t *x,
                                   const uint64_t *z, const uint32_t *ph, const char *arch, uint32_t *ngroups);
+
+/* ------------------------------------------------------------------ batched form
+ * Many circuits or noise trajectories, each run from |0...0> as
+ * QuantumState::run does (src/state/mod.rs:168-183 per circuit, with
+ * apply_op's Measure/Reset/Noise arms, mod.rs:134-165) and then evaluated on
+ * a list of Paulis (StateVector::expectation, vector.rs:412-429), all in one
+ * persistent launch per chunk of device memory.  Entries of up to 13 qubits
+ * (complex64; 12 for complex128) live in one workgroup's LDS for the whole
+ * circuit.
+ *
+ * Draws: every Measure, Reset and Noise op of an entry takes exactly one
+ * uniform in [0, 1), in program order (the single-state run skips the draw
+ * of a certain measurement; here the slot is taken either way).  A Noise op
+ * picks its error as Noise::sample does (circuit.rs:136-146).  Measurements
+ * must be Z-type Pauli products (M, MZ, MPP of Z factors) and gates of more
+ * than two qubits must be diagonal or permutations (CCZ, CCX): anything else
+ * is Unsupported.  A batch keeps pointers to its circuits until its next run
+ * has planned them. */
+typedef struct ncsv_batch ncsv_batch;
+
+ncsv_batch *ncsv_batch_new(ncsv_precision precision);
+void ncsv_batch_destroy(ncsv_batch *b);
+/* Adds a circuit with `count` Paulis (x, z, phase as in ncsv_expectation);
+ * *index = its entry number. */
+ncsv_status ncsv_batch_add(ncsv_batch *b, const ncsv_circuit *c, uint32_t count, const uint64_t *x,
+                           const uint64_t *z, const uint32_t *phase, uint32_t *index);
+uint32_t ncsv_batch_size(const ncsv_batch *b);
+/* The uniforms entry `entry` takes per run. */
+ncsv_status ncsv_batch_draws(const ncsv_batch *b, uint32_t entry, uint32_t *count);
+/* Tile size in qubits (6..13 complex64, 6..12 complex128; 0 = largest) and
+ * the persistent grid (0 = every resident workgroup). */
+ncsv_status ncsv_batch_set_tile_bits(ncsv_batch *b, uint32_t bits);
+ncsv_status ncsv_batch_set_blocks(ncsv_batch *b, int32_t blocks);
+/* Runs every entry.  `uniforms` holds each entry's draws back to back (in
+ * entry order), or is NULL and entry e's draws come from a SplitMix64 stream
+ * seeded with seeds[e] (gen_f64 = (next >> 11) * 2^-53).  `forced`, laid out
+ * like the uniforms (or NULL), fixes measurement outcomes: 0 or 1 forces,
+ * 0xff draws. */
+ncsv_status ncsv_batch_run(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced);
+/* Entry `entry` of the last run: expect[2k], expect[2k+1] = Re, Im <P_k>
+ * (unnormalized, as StateVector::expectation), *norm_sq = <psi|psi>, and
+ * outcomes[d] = the outcome taken at draw d (0xff for noise draws). */
+ncsv_status ncsv_batch_result(const ncsv_batch *b, uint32_t entry, double *expect, double *norm_sq,
+                              uint8_t *outcomes);
+/* out[8]: kernel ms, planning ms, tiles processed, launches, amplitude
+ * updates (gates x 2^n summed), HBM bytes of the multi-tile states, programs,
+ * program steps. */
+ncsv_status ncsv_batch_stats(const ncsv_batch *b, double *out);
+/* Plans every entry not planned yet (runs do this themselves; no device). */
+ncsv_status ncsv_batch_plan(ncsv_batch *b);
+/* Diagnostics: runs the same programs on FP64 host amplitudes (no device). */
+ncsv_status ncsv_batch_emulate(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced);
+/* Diagnostics: runs the device kernel's own code on the host, one workgroup
+ * simulated by host threads (no device; small batches only). */
+ncsv_status ncsv_batch_simulate(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced);
 
 #ifdef __cplusplus
 }

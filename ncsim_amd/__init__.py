@@ -445,3 +445,110 @@ class HipStateVector:
 
     def synchronize(self):
         _check(L.lib().ncsv_synchronize(self._h))
+
+
+class HipBatch:
+    """Many circuits or noise trajectories run by one persistent launch per
+    chunk of device memory (include/ncsv.h, batched form).  Each entry is a
+    circuit run from |0...0> plus Paulis evaluated at its end; every Measure,
+    Reset and Noise op takes one uniform per run, in program order."""
+
+    def __init__(self, precision=C128):
+        self._h = L.lib().ncsv_batch_new(int(precision))
+        if not self._h:
+            _check(8)
+        self._circuits = []  # kept alive with the batch
+        self._npauli = []
+        self._ndraws = []
+
+    def __del__(self):
+        if getattr(self, "_h", None) and L._lib is not None:
+            L._lib.ncsv_batch_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return L.lib().ncsv_batch_size(self._h)
+
+    def add(self, circuit, paulis=()):
+        k = len(paulis)
+        xs = (ctypes.c_uint64 * max(1, k))(*[_m64(p.x) for p in paulis])
+        zs = (ctypes.c_uint64 * max(1, k))(*[_m64(p.z) for p in paulis])
+        ph = (ctypes.c_uint32 * max(1, k))(*[p.phase & 3 for p in paulis])
+        idx = ctypes.c_uint32()
+        _check(L.lib().ncsv_batch_add(self._h, circuit._h, k, xs, zs, ph, ctypes.byref(idx)))
+        d = ctypes.c_uint32()
+        _check(L.lib().ncsv_batch_draws(self._h, idx.value, ctypes.byref(d)))
+        self._circuits.append(circuit)
+        self._npauli.append(k)
+        self._ndraws.append(d.value)
+        return idx.value
+
+    def draws(self, i):
+        return self._ndraws[i]
+
+    def set_tile_bits(self, bits):
+        _check(L.lib().ncsv_batch_set_tile_bits(self._h, int(bits)))
+
+    def set_blocks(self, blocks):
+        _check(L.lib().ncsv_batch_set_blocks(self._h, int(blocks)))
+
+    def _args(self, uniforms, seeds, forced):
+        import numpy as np
+        u = s = f = None
+        if uniforms is not None:
+            u = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in uniforms])
+                                     if isinstance(uniforms, (list, tuple)) else uniforms, dtype=np.float64)
+            if u.size != sum(self._ndraws):
+                raise NcsvError(8, f"{u.size} uniforms for {sum(self._ndraws)} draws")
+        if seeds is not None:
+            s = np.ascontiguousarray(seeds, dtype=np.uint64)
+            if s.size != len(self):
+                raise NcsvError(8, "one seed per entry")
+        if forced is not None:
+            f = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.uint8).ravel() for x in forced])
+                                     if isinstance(forced, (list, tuple)) else forced, dtype=np.uint8)
+            if f.size != sum(self._ndraws):
+                raise NcsvError(8, "forced outcomes are laid out like the uniforms")
+        keep = (u, s, f)
+        cu = u.ctypes.data_as(L.c_dp) if u is not None else None
+        cs = s.ctypes.data_as(L.c_u64p) if s is not None else None
+        cf = f.ctypes.data_as(L.c_u8p) if f is not None else None
+        return keep, cu, cs, cf
+
+    def plan(self):
+        """Plans every entry not planned yet (no device needed)."""
+        _check(L.lib().ncsv_batch_plan(self._h))
+
+    def run(self, uniforms=None, seeds=None, forced=None):
+        keep, cu, cs, cf = self._args(uniforms, seeds, forced)
+        _check(L.lib().ncsv_batch_run(self._h, cu, cs, cf))
+        del keep
+
+    def emulate(self, uniforms=None, seeds=None, forced=None):
+        """The same programs on FP64 host amplitudes (diagnostics, no device)."""
+        keep, cu, cs, cf = self._args(uniforms, seeds, forced)
+        _check(L.lib().ncsv_batch_emulate(self._h, cu, cs, cf))
+        del keep
+
+    def simulate(self, uniforms=None, seeds=None, forced=None):
+        """The device kernel's own code on the host, one workgroup simulated
+        by host threads (diagnostics, no device, small batches)."""
+        keep, cu, cs, cf = self._args(uniforms, seeds, forced)
+        _check(L.lib().ncsv_batch_simulate(self._h, cu, cs, cf))
+        del keep
+
+    def result(self, i):
+        """(expectations, norm_sq, outcomes per draw (None for noise draws))."""
+        k = self._npauli[i]
+        out = (ctypes.c_double * max(2, 2 * k))()
+        nn = ctypes.c_double()
+        oc = (ctypes.c_uint8 * max(1, self._ndraws[i]))()
+        _check(L.lib().ncsv_batch_result(self._h, i, out, ctypes.byref(nn), oc))
+        outc = [None if oc[d] == 0xff else int(oc[d]) for d in range(self._ndraws[i])]
+        return [complex(out[2 * j], out[2 * j + 1]) for j in range(k)], nn.value, outc
+
+    def stats(self):
+        out = (ctypes.c_double * 8)()
+        _check(L.lib().ncsv_batch_stats(self._h, out))
+        keys = ("kernel_ms", "plan_ms", "tiles", "launches", "amp_updates", "hbm_bytes", "programs", "steps")
+        return dict(zip(keys, list(out)))

@@ -73,6 +73,16 @@ _SIGS = {
     "ncsv_jit_check": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, c_u32p],
     "ncsv_jit_check_expect": [ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, c_u64p, c_u64p, c_u32p,
                               ctypes.c_char_p, c_u32p],
+    "ncsv_batch_add": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u64p, c_u32p, c_u32p],
+    "ncsv_batch_draws": [ctypes.c_void_p, ctypes.c_uint32, c_u32p],
+    "ncsv_batch_set_tile_bits": [ctypes.c_void_p, ctypes.c_uint32],
+    "ncsv_batch_set_blocks": [ctypes.c_void_p, ctypes.c_int32],
+    "ncsv_batch_run": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
+    "ncsv_batch_emulate": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
+    "ncsv_batch_simulate": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
+    "ncsv_batch_result": [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_u8p],
+    "ncsv_batch_stats": [ctypes.c_void_p, c_dp],
+    "ncsv_batch_plan": [ctypes.c_void_p],
 }
 
 _lib = None
@@ -95,10 +105,17 @@ def lib():
         L.ncsv_device_count.restype = ctypes.c_int
         L.ncsv_circuit_to_text.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         L.ncsv_circuit_to_text.restype = ctypes.c_size_t
+        L.ncsv_batch_new.argtypes = [ctypes.c_int]
+        L.ncsv_batch_new.restype = ctypes.c_void_p
+        L.ncsv_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.ncsv_batch_destroy.restype = None
+        L.ncsv_batch_size.argtypes = [ctypes.c_void_p]
+        L.ncsv_batch_size.restype = ctypes.c_uint32
         _lib = L
     return _lib
 
 
 def exported_symbols():
     """Names declared by include/ncsv.h (for the export check)."""
-    return sorted(list(_SIGS) + ["ncsv_last_error", "ncsv_device_count", "ncsv_circuit_to_text"])
+    return sorted(list(_SIGS) + ["ncsv_last_error", "ncsv_device_count", "ncsv_circuit_to_text", "ncsv_batch_new",
+                                 "ncsv_batch_destroy", "ncsv_batch_size"])

@@ -161,6 +161,11 @@ void do_noise(ncsv_state *s, const COp &op, ncsv_uniform_fn rng, void *ctx) {
 
 }  // namespace
 
+namespace ncsv {
+// The guard for entry points defined in other files (the batched form).
+ncsv_status guard_status(const std::function<void()> &f) { return guard(f); }
+}  // namespace ncsv
+
 extern "C" {
 
 size_t ncsv_last_error(char *buf, size_t cap) {

@@ -1,0 +1,1265 @@
+// Host side of the batched form: per-circuit programs (planned once and
+// shared by every entry running the same circuit), the queue of (entry,
+// step) records, chunking by device memory, the launch, and the FP64
+// results.  A host emulator runs the same programs on FP64 amplitudes so the
+// planner can be checked without a device.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <functional>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <future>
+#include <thread>
+
+#include "batch.hpp"
+#include "fuse.hpp"
+#include "ncsv_internal.hpp"
+
+namespace ncsv {
+namespace batch {
+
+uint32_t max_tile_bits(ncsv_precision prec);
+int resident_blocks(ncsv_precision prec);
+void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipStream_t stream);
+void simulate(const BArgs &A, ncsv_precision prec);
+
+struct StepH {
+    uint32_t L = 0;
+    std::vector<uint8_t> qt;  // tile qubits, then outer qubits ascending
+    std::vector<BOpD> ops;
+    std::vector<cd> coef;
+    std::vector<BPauliD> paulis;
+    uint32_t flags = 0;
+    bool modifies = false;
+};
+
+struct NoiseRef {
+    ncsv_noise ch;
+    double p[3];
+    uint32_t slot0, k;
+};
+
+// where one Pauli's per-tile partials live
+struct ExpRef {
+    uint64_t off;
+    uint32_t K, k, tiles;
+};
+
+struct Program {
+    uint32_t n = 0, L = 0;
+    std::vector<StepH> steps;
+    uint32_t nmeas = 0, nslots = 0;
+    uint64_t npart = 0, nexp = 0;     // partial doubles per entry
+    std::vector<uint8_t> draw_kind;   // 0 noise, 1 measure, 2 reset
+    std::vector<uint32_t> draw_ref;   // noise index or measurement ordinal
+    std::vector<NoiseRef> noise;
+    std::vector<ExpRef> exps;         // the Paulis, then the identity (norm)
+    uint64_t gates = 0;               // unfused gates
+    uint64_t items = 0;               // tiles over all steps
+    uint64_t hbm_elems = 0;           // amplitudes read + written over all steps
+    // placement in the device tables
+    uint32_t step0 = 0;
+    uint32_t tiles(uint32_t s) const { return 1u << (n - steps[s].L); }
+};
+
+namespace {
+
+uint64_t splitmix_next(uint64_t *s) {
+    *s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = *s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double splitmix_f64(uint64_t *s) { return (double)(splitmix_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// A Pauli gate as one-qubit tile ops: Z factors, then X factors, then i^ph
+// (apply_pauli's action, vector.rs:397-410).
+void expand_pauli(const LOp &p, std::vector<LOp> *out) {
+    for (uint64_t a = p.z; a; a &= a - 1) {
+        LOp d;
+        d.kind = OpKind::Diag;
+        d.qubits = {(uint32_t)__builtin_ctzll(a)};
+        d.m = {cd(1, 0), cd(-1, 0)};
+        out->push_back(d);
+    }
+    for (uint64_t a = p.x; a; a &= a - 1) {
+        LOp d;
+        d.kind = OpKind::Perm;
+        d.qubits = {(uint32_t)__builtin_ctzll(a)};
+        d.m = {cd(1, 0), cd(1, 0)};
+        d.img = {1, 0};
+        out->push_back(d);
+    }
+    if (p.ph & 3) {
+        static const cd ip[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
+        LOp d;
+        d.kind = OpKind::Diag;
+        d.m = {ip[p.ph & 3]};
+        out->push_back(d);
+    }
+}
+
+struct Ev {
+    bool meas = false;
+    std::vector<LOp> ops;
+    bool reset = false;
+    uint64_t z = 0;
+    uint32_t ph = 0, q = 0, ord = 0;
+};
+
+int pos_in(const StepH &s, uint32_t q) {
+    for (uint32_t p = 0; p < s.L; p++)
+        if (s.qt[p] == q) return (int)p;
+    return -1;
+}
+
+void split_mask(const StepH &s, uint64_t m, uint32_t *lm, uint64_t *om) {
+    *lm = 0;
+    *om = 0;
+    for (uint64_t a = m; a; a &= a - 1) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        const int p = pos_in(s, q);
+        if (p >= 0)
+            *lm |= 1u << p;
+        else
+            *om |= 1ull << q;
+    }
+}
+
+void append_tile_ops(StepH *s, const SweepH &sw) {
+    for (const TileOpH &o : sw.ops) {
+        BOpD d{};
+        d.p0 = o.j0;
+        d.p1 = o.j1;
+        d.k0 = o.k0;
+        d.k1 = o.k1;
+        d.lmask = o.rmask;
+        d.omask = o.gmask;
+        auto put = [&](int cnt) {
+            d.coef = (uint32_t)s->coef.size();
+            for (int t = 0; t < cnt; t++) s->coef.push_back(o.m[t]);
+        };
+        switch (o.kind) {
+        case TK_U1: d.kind = BK_U1; put(4); break;
+        case TK_U1R: d.kind = BK_U1R; put(4); break;
+        case TK_U1X: d.kind = BK_U1X; put(4); break;
+        case TK_U2: d.kind = BK_U2; put(16); break;
+        case TK_DM: d.kind = BK_DM; put(1); break;
+        case TK_PERM1: d.kind = BK_PERM1; break;
+        case TK_SWAPR: d.kind = BK_SWAP; break;
+        case TK_NSLOT: d.kind = BK_NSLOT; d.a = o.aux; break;
+        default: fail(NCSV_INVALID, "batched form: unexpected tile op");
+        }
+        s->ops.push_back(d);
+    }
+}
+
+}  // namespace
+
+// Plans one circuit (plus its Paulis) for tiles of at most Lb qubits.
+std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<uint64_t> &X,
+                                       const std::vector<uint64_t> &Z, const std::vector<uint32_t> &PH,
+                                       ncsv_precision prec, uint32_t Lb) {
+    auto P = std::make_shared<Program>();
+    const uint32_t n = c->n;
+    P->n = n;
+    const uint32_t L = std::min(n, Lb);
+    P->L = L;
+    // low qubits fixed in every gate tile (256-byte runs), leaving room for a
+    // two-qubit gate plus a reset qubit on small tiles
+    uint32_t cl = std::min(prec == NCSV_C64 ? 5u : 4u, L > 3 ? L - 3 : 0u);
+    if (const char *e = getenv("NCSV_BATCH_LOW")) cl = std::min((uint32_t)atoi(e), cl);
+
+    // the circuit as gate runs and measurement events (ticks and annotations dropped)
+    std::vector<Ev> evs;
+    auto run = [&]() -> std::vector<LOp> & {
+        if (evs.empty() || evs.back().meas) evs.emplace_back();
+        return evs.back().ops;
+    };
+    for (const COp &op : c->ops) {
+        switch (op.kind) {
+        case CKind::Gate: {
+            LOp lo = lower_gate(op.g, op.qubits.data(), (uint32_t)op.qubits.size(), n);
+            P->gates++;
+            if (lo.kind == OpKind::Pauli)
+                expand_pauli(lo, &run());
+            else if (lo.kind != OpKind::Nop)
+                run().push_back(std::move(lo));
+            break;
+        }
+        case CKind::Noise: {
+            NoiseRef nr;
+            nr.ch = op.ch;
+            memcpy(nr.p, op.p, sizeof nr.p);
+            nr.slot0 = P->nslots;
+            nr.k = (uint32_t)op.qubits.size();
+            for (uint32_t j = 0; j < nr.k; j++) {
+                LOp s;
+                s.kind = OpKind::Slot;
+                s.qubits = {op.qubits[j]};
+                s.ph = P->nslots + j;
+                run().push_back(s);
+            }
+            P->nslots += nr.k;
+            P->draw_kind.push_back(0);
+            P->draw_ref.push_back((uint32_t)P->noise.size());
+            P->noise.push_back(nr);
+            break;
+        }
+        case CKind::Measure:
+        case CKind::Reset: {
+            Ev e;
+            e.meas = true;
+            if (op.kind == CKind::Reset) {
+                e.reset = true;
+                e.q = op.qubits[0];
+                e.z = 1ull << e.q;
+            } else {
+                if (op.x) fail(NCSV_UNSUPPORTED, "batched form: measurements of Paulis with X or Y factors");
+                if (op.ph & 1) fail(NCSV_INVALID, "measured operator is not Hermitian");
+                e.z = op.z;
+                e.ph = op.ph & 3;
+            }
+            e.ord = P->nmeas++;
+            P->draw_kind.push_back(e.reset ? 2 : 1);
+            P->draw_ref.push_back(e.ord);
+            evs.push_back(std::move(e));
+            break;
+        }
+        default: break;
+        }
+    }
+
+    // Pauli groups: up to 9 per EXPECT op, x-supports (plus the low qubits) within a tile
+    std::vector<BPauliD> dummy;
+    struct Pg {
+        std::vector<uint32_t> idx;  // into the Pauli list (npauli = identity)
+        uint64_t xs = 0;
+    };
+    const uint32_t np = (uint32_t)X.size();
+    std::vector<Pg> groups;
+    {
+        for (uint32_t k = 0; k <= np; k++) {
+            const uint64_t xs = k < np ? X[k] : 0;
+            bool placed = false;
+            for (Pg &g : groups) {
+                if (g.idx.size() >= 9) continue;
+                if ((uint32_t)__builtin_popcountll(g.xs | xs) > L) continue;
+                g.idx.push_back(k);
+                g.xs |= xs;
+                placed = true;
+                break;
+            }
+            if (!placed) {
+                if ((uint32_t)__builtin_popcountll(xs) > L)
+                    fail(NCSV_UNSUPPORTED, "batched form: a Pauli's X/Y support does not fit a tile");
+                Pg g;
+                g.idx.push_back(k);
+                g.xs = xs;
+                groups.push_back(g);
+            }
+        }
+    }
+    P->exps.resize(np + 1);
+    auto add_expect = [&](StepH &s, const Pg &g, uint32_t tiles) {
+        BOpD d{};
+        d.kind = BK_EXPECT;
+        d.a = (uint32_t)P->nexp;
+        d.b = (uint32_t)g.idx.size();
+        d.c = (uint32_t)s.paulis.size();
+        for (uint32_t kk = 0; kk < g.idx.size(); kk++) {
+            const uint32_t k = g.idx[kk];
+            BPauliD pd{};
+            if (k < np) {
+                uint32_t xl = 0, zl = 0;
+                uint64_t xo = 0, zo = 0;
+                split_mask(s, X[k], &xl, &xo);
+                split_mask(s, Z[k], &zl, &zo);
+                if (xo) fail(NCSV_INVALID, "batched planner: Pauli X support outside the tile");
+                pd.xl = xl;
+                pd.zl = zl;
+                pd.zo = zo;
+                pd.ph = PH[k] & 3;
+            }
+            s.paulis.push_back(pd);
+            P->exps[k] = ExpRef{P->nexp, (uint32_t)g.idx.size(), kk, tiles};
+        }
+        P->nexp += (uint64_t)2 * g.idx.size() * tiles;
+        s.ops.push_back(d);
+    };
+
+    if (n <= Lb) {
+        // one step on one tile: the state stays in LDS
+        StepH s;
+        s.L = n;
+        for (uint32_t q = 0; q < n; q++) s.qt.push_back((uint8_t)q);
+        s.flags = BF_INIT;
+        s.modifies = true;
+        for (const Ev &e : evs) {
+            if (!e.meas) {
+                if (e.ops.empty()) continue;
+                for (const SweepH &sw : plan_lds_sweeps(e.ops, n, n, cl, 0)) {
+                    for (uint32_t p = 0; p < n; p++)
+                        if (sw.tq[p] != p) fail(NCSV_INVALID, "batched planner: whole-state tile not in order");
+                    append_tile_ops(&s, sw);
+                }
+            } else {
+                BOpD d{};
+                d.kind = BK_MEASURE;
+                d.lmask = (uint32_t)e.z;
+                d.k0 = (uint8_t)(e.ph >> 1);
+                d.c = e.ord;
+                d.p0 = e.reset ? (uint8_t)e.q : (uint8_t)0xff;
+                s.ops.push_back(d);
+            }
+        }
+        P->steps.push_back(std::move(s));
+        for (const Pg &g : groups) add_expect(P->steps.back(), g, 1);
+    } else {
+        struct Pend {
+            bool on = false;
+            Ev ev;
+            uint32_t off = 0, tiles = 0;
+        } pend;
+        const uint32_t tiles = 1u << (n - L);
+        // the needed qubits, filled up from qubit 0 (the low qubits keep loads coalesced)
+        auto default_tq = [&](uint64_t need) {
+            uint64_t T = need;
+            for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(T) < L; q++) T |= 1ull << q;
+            std::vector<uint32_t> tq;
+            for (uint64_t a = T; a; a &= a - 1) tq.push_back((uint32_t)__builtin_ctzll(a));
+            return tq;
+        };
+        auto make_step = [&](const std::vector<uint32_t> &tq) -> StepH & {
+            StepH s;
+            s.L = L;
+            uint64_t in = 0;
+            for (uint32_t q : tq) {
+                s.qt.push_back((uint8_t)q);
+                in |= 1ull << q;
+            }
+            for (uint32_t q = 0; q < n; q++)
+                if (!((in >> q) & 1)) s.qt.push_back((uint8_t)q);
+            if (s.qt.size() != n || tq.size() != L) fail(NCSV_INVALID, "batched planner: bad tile");
+            s.flags = P->steps.empty() ? BF_INIT : BF_LOAD;
+            s.modifies = P->steps.empty();
+            if (pend.on) {
+                BOpD d{};
+                d.kind = BK_COLLAPSE;
+                d.a = pend.off;
+                d.b = pend.tiles;
+                d.c = pend.ev.ord;
+                split_mask(s, pend.ev.z, &d.lmask, &d.omask);
+                d.k0 = (uint8_t)(pend.ev.ph >> 1);
+                d.p0 = 0xff;
+                if (pend.ev.reset) {
+                    const int p = pos_in(s, pend.ev.q);
+                    if (p < 0) fail(NCSV_INVALID, "batched planner: reset qubit outside the tile");
+                    d.p0 = (uint8_t)p;
+                }
+                s.ops.push_back(d);
+                s.modifies = true;
+                pend.on = false;
+            }
+            P->steps.push_back(std::move(s));
+            return P->steps.back();
+        };
+        for (const Ev &e : evs) {
+            if (!e.meas) {
+                if (e.ops.empty()) continue;
+                const uint64_t ff = (pend.on && pend.ev.reset) ? 1ull << pend.ev.q : 0;
+                for (const SweepH &sw : plan_lds_sweeps(e.ops, n, L, cl, ff)) {
+                    StepH &s = make_step(sw.tq);
+                    append_tile_ops(&s, sw);
+                    s.modifies = true;
+                }
+            } else {
+                if (P->steps.empty() || pend.on)
+                    make_step(default_tq(pend.on && pend.ev.reset ? 1ull << pend.ev.q : 0));
+                StepH &s = P->steps.back();
+                BOpD d{};
+                d.kind = BK_REDUCE;
+                d.a = (uint32_t)P->npart;
+                split_mask(s, e.z, &d.lmask, &d.omask);
+                d.k0 = (uint8_t)(e.ph >> 1);
+                s.ops.push_back(d);
+                pend.on = true;
+                pend.ev = e;
+                pend.off = (uint32_t)P->npart;
+                pend.tiles = tiles;
+                P->npart += 2ull * tiles;
+            }
+        }
+        auto fits_last = [&](uint64_t xs) {
+            if (pend.on || P->steps.empty()) return false;
+            for (uint64_t a = xs; a; a &= a - 1)
+                if (pos_in(P->steps.back(), (uint32_t)__builtin_ctzll(a)) < 0) return false;
+            return true;
+        };
+        for (const Pg &g : groups) {
+            if (!fits_last(g.xs)) {
+                uint64_t need = g.xs;
+                if (pend.on && pend.ev.reset) {
+                    const uint64_t rq = 1ull << pend.ev.q;
+                    if ((uint32_t)__builtin_popcountll(need | rq) > L)
+                        make_step(default_tq(rq));  // the reset's collapse on its own
+                    else
+                        need |= rq;
+                }
+                if (!fits_last(g.xs)) make_step(default_tq(need));
+            }
+            add_expect(P->steps.back(), g, tiles);
+        }
+        for (size_t i = 0; i + 1 < P->steps.size(); i++)
+            if (P->steps[i].modifies) P->steps[i].flags |= BF_STORE;
+    }
+    for (uint32_t s = 0; s < P->steps.size(); s++) {
+        P->items += P->tiles(s);
+        if (n > Lb) {
+            const uint64_t N = 1ull << n;
+            if (P->steps[s].flags & BF_LOAD) P->hbm_elems += N;
+            if (P->steps[s].flags & BF_STORE) P->hbm_elems += N;
+        }
+    }
+    return P;
+}
+
+}  // namespace batch
+}  // namespace ncsv
+
+using ncsv::cd;
+using namespace ncsv::batch;
+
+struct ncsv_batch {
+    ncsv_precision prec = NCSV_C128;
+    uint32_t tile_bits = 0;  // 0: the largest tile for the precision
+    int blocks = 0;          // 0: every resident workgroup
+    struct Entry {
+        const ncsv_circuit *c = nullptr;
+        std::vector<uint64_t> x, z;
+        std::vector<uint32_t> ph;
+        int prog = -1;
+        uint32_t ndraws = 0;
+    };
+    std::vector<Entry> entries;
+    std::vector<std::shared_ptr<Program>> progs;
+    uint32_t planned_bits = 0;
+    // device program tables
+    void *d_tab = nullptr;
+    size_t tab_bytes = 0;
+    size_t off_steps = 0, off_ops = 0, off_coef = 0, off_paulis = 0, off_qtab = 0;
+    bool tables_dirty = true;
+    // per-chunk device buffers (grow only)
+    void *d_amp = nullptr;
+    size_t amp_cap = 0;
+    void *d_work = nullptr;
+    size_t work_cap = 0;
+    // results of the last run
+    std::vector<std::vector<double>> expv;
+    std::vector<double> norm;
+    std::vector<std::vector<uint8_t>> outc;
+    double kernel_ms = 0.0, plan_ms = 0.0;
+    uint64_t items = 0, launches = 0;
+    double amp_updates = 0.0, hbm_bytes = 0.0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+};
+
+namespace ncsv {
+namespace batch {
+namespace {
+
+uint32_t count_draws(const ncsv_circuit *c) {
+    uint32_t d = 0;
+    for (const COp &op : c->ops)
+        if (op.kind == CKind::Measure || op.kind == CKind::Reset || op.kind == CKind::Noise) d++;
+    return d;
+}
+
+uint32_t tile_bits_of(const ncsv_batch *b) {
+    const uint32_t mx = max_tile_bits(b->prec);
+    if (b->tile_bits) return std::min(b->tile_bits, mx);
+    if (const char *e = getenv("NCSV_BATCH_TILE_BITS")) {
+        const int v = atoi(e);
+        if (v >= 6) return std::min((uint32_t)v, mx);
+    }
+    return mx;
+}
+
+// Plans every entry's program (entries running the same circuit with the
+// same Paulis share one), in parallel host threads.
+void plan_all(ncsv_batch *b) {
+    const uint32_t Lb = tile_bits_of(b);
+    if (Lb != b->planned_bits) {
+        for (auto &e : b->entries) e.prog = -1;
+        b->progs.clear();
+        b->planned_bits = Lb;
+        b->tables_dirty = true;
+    }
+    std::vector<size_t> todo;
+    for (size_t i = 0; i < b->entries.size(); i++) {
+        auto &e = b->entries[i];
+        if (e.prog >= 0) continue;
+        // share with an earlier entry of the same circuit and Paulis
+        for (size_t j = 0; j < i; j++) {
+            const auto &f = b->entries[j];
+            if (f.c == e.c && f.x == e.x && f.z == e.z && f.ph == e.ph && (f.prog >= 0 || j < i)) {
+                e.prog = f.prog >= 0 ? f.prog : -2 - (int)j;
+                break;
+            }
+        }
+        if (e.prog == -1) todo.push_back(i);
+    }
+    if (todo.empty() && std::all_of(b->entries.begin(), b->entries.end(), [](const ncsv_batch::Entry &e) {
+            return e.prog >= 0;
+        }))
+        return;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::shared_ptr<Program>> made(todo.size());
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, todo.size());
+    std::vector<std::future<void>> fs;
+    std::atomic<size_t> next{0};
+    for (unsigned t = 0; t < nt; t++) {
+        fs.push_back(std::async(std::launch::async, [&]() {
+            for (;;) {
+                const size_t k = next.fetch_add(1);
+                if (k >= todo.size()) return;
+                const auto &e = b->entries[todo[k]];
+                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb);
+            }
+        }));
+    }
+    for (auto &f : fs) f.get();  // rethrows a planning error
+    for (size_t k = 0; k < todo.size(); k++) {
+        b->entries[todo[k]].prog = (int)b->progs.size();
+        b->progs.push_back(made[k]);
+    }
+    for (auto &e : b->entries)
+        if (e.prog <= -2) e.prog = b->entries[(size_t)(-2 - e.prog)].prog;
+    b->tables_dirty = true;
+    b->plan_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T> T *at(void *base, size_t off) { return (T *)((char *)base + off); }
+size_t up256(size_t v) { return (v + 255) / 256 * 256; }
+
+// The programs' steps, ops, coefficients, Paulis and qubit tables, uploaded once.
+struct Tables {
+    std::vector<BStepD> steps;
+    std::vector<BOpD> ops;
+    std::vector<double> coef;
+    std::vector<BPauliD> paulis;
+    std::vector<uint8_t> qtab;
+};
+
+Tables flatten(ncsv_batch *b) {
+    Tables t;
+    for (auto &P : b->progs) {
+        P->step0 = (uint32_t)t.steps.size();
+        for (const StepH &s : P->steps) {
+            BStepD d{};
+            d.L = s.L;
+            d.n = P->n;
+            d.op0 = (uint32_t)t.ops.size();
+            d.nops = (uint32_t)s.ops.size();
+            d.flags = s.flags;
+            d.qoff = (uint32_t)t.qtab.size();
+            const uint32_t c0 = (uint32_t)(t.coef.size() / 2), p0 = (uint32_t)t.paulis.size();
+            for (BOpD o : s.ops) {
+                if (o.kind <= BK_DM) o.coef += c0;
+                if (o.kind == BK_EXPECT) o.c += p0;
+                t.ops.push_back(o);
+            }
+            for (const cd &c : s.coef) {
+                t.coef.push_back(c.real());
+                t.coef.push_back(c.imag());
+            }
+            t.paulis.insert(t.paulis.end(), s.paulis.begin(), s.paulis.end());
+            t.qtab.insert(t.qtab.end(), s.qt.begin(), s.qt.end());
+            t.steps.push_back(d);
+        }
+    }
+    return t;
+}
+
+void upload_tables(ncsv_batch *b, const Tables &t) {
+    size_t off = 0;
+    b->off_steps = off;
+    off = up256(off + t.steps.size() * sizeof(BStepD));
+    b->off_ops = off;
+    off = up256(off + t.ops.size() * sizeof(BOpD));
+    b->off_coef = off;
+    off = up256(off + t.coef.size() * sizeof(double));
+    b->off_paulis = off;
+    off = up256(off + t.paulis.size() * sizeof(BPauliD));
+    b->off_qtab = off;
+    off = up256(off + t.qtab.size() + 1);
+    if (off > b->tab_bytes) {
+        if (b->d_tab) check_hip(hipFree(b->d_tab), "hipFree");
+        b->d_tab = nullptr;
+        check_hip(hipMalloc(&b->d_tab, off), "hipMalloc(batch programs)");
+        b->tab_bytes = off;
+    }
+    std::vector<char> h(off, 0);
+    memcpy(h.data() + b->off_steps, t.steps.data(), t.steps.size() * sizeof(BStepD));
+    memcpy(h.data() + b->off_ops, t.ops.data(), t.ops.size() * sizeof(BOpD));
+    memcpy(h.data() + b->off_coef, t.coef.data(), t.coef.size() * sizeof(double));
+    memcpy(h.data() + b->off_paulis, t.paulis.data(), t.paulis.size() * sizeof(BPauliD));
+    memcpy(h.data() + b->off_qtab, t.qtab.data(), t.qtab.size());
+    check_hip(hipMemcpy(b->d_tab, h.data(), off, hipMemcpyHostToDevice), "upload batch programs");
+    b->tables_dirty = false;
+}
+
+// One chunk's host-side arrays.
+struct Chunk {
+    std::vector<uint32_t> ent;  // batch entry of each chunk entry
+    std::vector<BEntryD> ed;
+    std::vector<BRecD> recs;
+    std::vector<uint32_t> rec_blk;
+    uint64_t total = 0, amp_elems = 0, npart = 0, nexp = 0;
+    uint32_t nmeas = 0, nslots = 0;
+    std::vector<double> uni, lsc;
+    std::vector<uint8_t> forced, errs, outc;
+    bool any_forced = false;
+    std::vector<double> pexp;
+};
+
+// Draws of one entry: uniforms from the caller's array or its SplitMix64
+// seed, in program order; noise draws become per-slot error codes
+// (Noise::sample, circuit.rs:136-146, with the error applied as raw x/z bits).
+void fill_draws(const Program &P, const double *u, uint64_t *seed, const uint8_t *forced, Chunk *ch,
+                const BEntryD &ed) {
+    for (size_t d = 0; d < P.draw_kind.size(); d++) {
+        double r = 0.0;
+        if (u)
+            r = u[d];
+        else if (seed)
+            r = splitmix_f64(seed);
+        if (P.draw_kind[d] == 0) {
+            const NoiseRef &nr = P.noise[P.draw_ref[d]];
+            for (const NoiseErr &e : noise_errors(nr.ch, nr.p)) {
+                if (r < e.p) {
+                    for (uint32_t j = 0; j < nr.k; j++)
+                        ch->errs[ed.err0 + nr.slot0 + j] = (uint8_t)(((e.x >> j) & 1) | (((e.z >> j) & 1) << 1));
+                    break;
+                }
+                r -= e.p;
+            }
+        } else {
+            const uint32_t o = ed.m0 + P.draw_ref[d];
+            ch->uni[o] = r;
+            if (forced && forced[d] != 0xff) {
+                ch->forced[o] = forced[d] & 1;
+                ch->any_forced = true;
+            }
+        }
+    }
+}
+
+Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *uniforms,
+                 const std::vector<size_t> &draw_off, const uint64_t *seeds, const uint8_t *forced) {
+    Chunk ch;
+    ch.ent = ent;
+    const size_t ne = ent.size();
+    ch.ed.resize(ne);
+    for (size_t i = 0; i < ne; i++) {
+        const Program &P = *b->progs[b->entries[ent[i]].prog];
+        BEntryD &d = ch.ed[i];
+        d.amp_off = ch.amp_elems;
+        if (P.n > P.L) ch.amp_elems += 1ull << P.n;
+        d.part0 = ch.npart;
+        ch.npart += P.npart;
+        d.exp0 = ch.nexp;
+        ch.nexp += P.nexp;
+        d.m0 = ch.nmeas;
+        ch.nmeas += P.nmeas;
+        d.err0 = ch.nslots;
+        ch.nslots += P.nslots;
+    }
+    ch.uni.assign(ch.nmeas + 1, 0.0);
+    ch.forced.assign(ch.nmeas + 1, 0xff);
+    ch.errs.assign(ch.nslots + 1, 0);
+    ch.lsc.assign(ne, 1.0);
+    for (size_t i = 0; i < ne; i++) {
+        const uint32_t e = ent[i];
+        const Program &P = *b->progs[b->entries[e].prog];
+        uint64_t seed = seeds ? seeds[e] : 0;
+        fill_draws(P, uniforms ? uniforms + draw_off[e] : nullptr, seeds ? &seed : nullptr,
+                   forced ? forced + draw_off[e] : nullptr, &ch, ch.ed[i]);
+    }
+    // depth-major queue, deeper programs first within a depth
+    std::vector<uint32_t> order(ne);
+    for (uint32_t i = 0; i < ne; i++) order[i] = i;
+    auto nsteps = [&](uint32_t i) { return (uint32_t)b->progs[b->entries[ent[i]].prog]->steps.size(); };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return nsteps(a) > nsteps(c); });
+    std::vector<uint32_t> last(ne, ~0u);
+    const uint32_t maxd = ne ? nsteps(order[0]) : 0;
+    for (uint32_t d = 0; d < maxd; d++) {
+        for (uint32_t i : order) {
+            const Program &P = *b->progs[b->entries[ent[i]].prog];
+            if (d >= P.steps.size()) break;  // sorted by depth
+            BRecD r{};
+            r.first = ch.total;
+            r.entry = i;
+            r.step = P.step0 + d;
+            r.prev = last[i];
+            r.prev_tiles = d ? P.tiles(d - 1) : 0;
+            last[i] = (uint32_t)ch.recs.size();
+            ch.recs.push_back(r);
+            ch.total += P.tiles(d);
+        }
+    }
+    BRecD sentinel{};
+    sentinel.first = ch.total;
+    sentinel.prev = ~0u;
+    ch.recs.push_back(sentinel);
+    ch.rec_blk.assign((ch.total + 15) / 16 + 1, 0);
+    for (uint32_t r = 0; r + 1 < ch.recs.size(); r++)
+        for (uint64_t blk = (ch.recs[r].first + 15) / 16; blk * 16 < ch.recs[r + 1].first; blk++) ch.rec_blk[blk] = r;
+    return ch;
+}
+
+void ensure(void **p, size_t *cap, size_t bytes, const char *what) {
+    if (bytes <= *cap) return;
+    if (*p) check_hip(hipFree(*p), "hipFree");
+    *p = nullptr;
+    *cap = 0;
+    check_hip(hipMalloc(p, bytes), what);
+    *cap = bytes;
+}
+
+void run_chunk_device(ncsv_batch *b, Chunk &ch) {
+    const size_t es = b->prec == NCSV_C128 ? 16 : 8;
+    ensure(&b->d_amp, &b->amp_cap, std::max<size_t>(256, ch.amp_elems * es), "hipMalloc(batch states)");
+    // work buffer layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = up256(off + bytes);
+        return o;
+    };
+    const size_t o_ent = take(ch.ed.size() * sizeof(BEntryD)), o_rec = take(ch.recs.size() * sizeof(BRecD)),
+                 o_blk = take(ch.rec_blk.size() * 4), o_part = take((ch.npart + 1) * 8),
+                 o_exp = take((ch.nexp + 1) * 8), o_lsc = take(ch.lsc.size() * 8), o_uni = take(ch.uni.size() * 8),
+                 o_forced = take(ch.forced.size()), o_outc = take(ch.nmeas + 1), o_errs = take(ch.errs.size()),
+                 o_done = take(ch.recs.size() * 4), o_next = take(8), o_flag = take(4);
+    ensure(&b->d_work, &b->work_cap, off, "hipMalloc(batch work)");
+    char *w = (char *)b->d_work;
+    hipStream_t st = b->stream;
+    auto up = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) check_hip(hipMemcpyAsync(w + o, src, bytes, hipMemcpyHostToDevice, st), "upload batch");
+    };
+    up(o_ent, ch.ed.data(), ch.ed.size() * sizeof(BEntryD));
+    up(o_rec, ch.recs.data(), ch.recs.size() * sizeof(BRecD));
+    up(o_blk, ch.rec_blk.data(), ch.rec_blk.size() * 4);
+    up(o_lsc, ch.lsc.data(), ch.lsc.size() * 8);
+    up(o_uni, ch.uni.data(), ch.uni.size() * 8);
+    up(o_forced, ch.forced.data(), ch.forced.size());
+    up(o_errs, ch.errs.data(), ch.errs.size());
+    BArgs A{};
+    A.steps = at<const BStepD>(b->d_tab, b->off_steps);
+    A.ops = at<const BOpD>(b->d_tab, b->off_ops);
+    A.coef = at<const double>(b->d_tab, b->off_coef);
+    A.paulis = at<const BPauliD>(b->d_tab, b->off_paulis);
+    A.qtab = at<const uint8_t>(b->d_tab, b->off_qtab);
+    A.entries = (const BEntryD *)(w + o_ent);
+    A.recs = (const BRecD *)(w + o_rec);
+    A.rec_blk = (const uint32_t *)(w + o_blk);
+    A.total = ch.total;
+    A.amp = b->d_amp;
+    A.part = (double *)(w + o_part);
+    A.pexp = (double *)(w + o_exp);
+    A.lsc = (double *)(w + o_lsc);
+    A.uni = (const double *)(w + o_uni);
+    A.forced = ch.any_forced ? (const uint8_t *)(w + o_forced) : nullptr;
+    A.outc = (uint8_t *)(w + o_outc);
+    A.errs = (const uint8_t *)(w + o_errs);
+    A.done = (uint32_t *)(w + o_done);
+    A.next = (unsigned long long *)(w + o_next);
+    A.flag = (uint32_t *)(w + o_flag);
+    A.spin_limit = 1u << 22;  // about a second of waiting per item
+    int blocks = b->blocks > 0 ? b->blocks : resident_blocks(b->prec);
+    if ((uint64_t)blocks > ch.total) blocks = (int)std::max<uint64_t>(1, ch.total);
+    check_hip(hipEventRecord(b->ev0, st), "hipEventRecord");
+    launch(A, (uint32_t)ch.recs.size(), b->prec, blocks, st);
+    check_hip(hipEventRecord(b->ev1, st), "hipEventRecord");
+    ch.pexp.assign(ch.nexp + 1, 0.0);
+    ch.outc.assign(ch.nmeas + 1, 0xff);
+    uint32_t flag = 0;
+    check_hip(hipMemcpyAsync(ch.pexp.data(), w + o_exp, ch.pexp.size() * 8, hipMemcpyDeviceToHost, st), "download");
+    check_hip(hipMemcpyAsync(ch.lsc.data(), w + o_lsc, ch.lsc.size() * 8, hipMemcpyDeviceToHost, st), "download");
+    check_hip(hipMemcpyAsync(ch.outc.data(), w + o_outc, ch.outc.size(), hipMemcpyDeviceToHost, st), "download");
+    check_hip(hipMemcpyAsync(&flag, w + o_flag, 4, hipMemcpyDeviceToHost, st), "download");
+    check_hip(hipStreamSynchronize(st), "batch run");
+    float ms = 0.0f;
+    check_hip(hipEventElapsedTime(&ms, b->ev0, b->ev1), "hipEventElapsedTime");
+    b->kernel_ms += ms;
+    b->launches++;
+    b->items += ch.total;
+    if (flag & 1) fail(NCSV_DEVICE, "batched run: a step waited too long for its predecessor");
+}
+
+// The kernel body itself on host memory (one simulated workgroup).
+void run_chunk_sim(ncsv_batch *b, const Tables &t, Chunk &ch) {
+    const size_t es = b->prec == NCSV_C128 ? 16 : 8;
+    std::vector<double> amp(std::max<size_t>(2, ch.amp_elems * es / 8), 0.0);
+    std::vector<double> part(ch.npart + 1, 0.0);
+    ch.pexp.assign(ch.nexp + 1, 0.0);
+    ch.outc.assign(ch.nmeas + 1, 0xff);
+    std::vector<uint32_t> done(ch.recs.size(), 0);
+    unsigned long long next = 0;
+    uint32_t flag = 0;
+    BArgs A{};
+    A.steps = t.steps.data();
+    A.ops = t.ops.data();
+    A.coef = t.coef.data();
+    A.paulis = t.paulis.data();
+    A.qtab = t.qtab.data();
+    A.entries = ch.ed.data();
+    A.recs = ch.recs.data();
+    A.rec_blk = ch.rec_blk.data();
+    A.total = ch.total;
+    A.amp = amp.data();
+    A.part = part.data();
+    A.pexp = ch.pexp.data();
+    A.lsc = ch.lsc.data();
+    A.uni = ch.uni.data();
+    A.forced = ch.any_forced ? ch.forced.data() : nullptr;
+    A.outc = ch.outc.data();
+    A.errs = ch.errs.data();
+    A.done = done.data();
+    A.next = &next;
+    A.flag = &flag;
+    A.spin_limit = 1u << 20;
+    simulate(A, b->prec);
+    b->launches++;
+    b->items += ch.total;
+    if (flag & 1) fail(NCSV_DEVICE, "batched simulation: a step waited too long for its predecessor");
+}
+
+// ---------------------------------------------------------------- emulator
+// The kernel's op semantics on FP64 host amplitudes.
+using cdv = std::vector<cd>;
+
+cd ipow_c(uint32_t k, cd v) {
+    switch (k & 3) {
+    case 1: return cd(-v.imag(), v.real());
+    case 2: return -v;
+    case 3: return cd(v.imag(), -v.real());
+    default: return v;
+    }
+}
+uint32_t ins0h(uint32_t j, uint32_t p) {
+    const uint32_t lo = j & ((1u << p) - 1);
+    return ((j ^ lo) << 1) | lo;
+}
+uint32_t parity(uint64_t v) { return (uint32_t)__builtin_popcountll(v) & 1u; }
+
+void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
+    ch.pexp.assign(ch.nexp + 1, 0.0);
+    ch.outc.assign(ch.nmeas + 1, 0xff);
+    std::vector<double> part(ch.npart + 1, 0.0);
+    const bool c64 = b->prec == NCSV_C64;
+    for (size_t i = 0; i < ch.ent.size(); i++) {
+        const Program &P = *b->progs[b->entries[ch.ent[i]].prog];
+        const BEntryD &en = ch.ed[i];
+        const uint32_t n = P.n;
+        cdv amp(size_t(1) << n, cd(0, 0));
+        for (uint32_t si = 0; si < P.steps.size(); si++) {
+            const BStepD &st = t.steps[P.step0 + si];
+            const uint32_t L = st.L, NL = 1u << L;
+            const uint8_t *qt = t.qtab.data() + st.qoff;
+            for (uint64_t tileno = 0; tileno < (1ull << (n - L)); tileno++) {
+                uint64_t base = 0;
+                for (uint32_t bb = 0; bb + L < n; bb++)
+                    if ((tileno >> bb) & 1) base |= 1ull << qt[L + bb];
+                std::vector<uint64_t> addr(NL);
+                for (uint32_t x = 0; x < NL; x++) {
+                    uint64_t off = base;
+                    for (uint32_t p = 0; p < L; p++)
+                        if ((x >> p) & 1) off |= 1ull << qt[p];
+                    addr[x] = off;
+                }
+                cdv tl(NL);
+                for (uint32_t x = 0; x < NL; x++)
+                    tl[x] = (st.flags & BF_INIT) ? cd((x == 0 && tileno == 0) ? 1.0 : 0.0, 0.0) : amp[addr[x]];
+                for (uint32_t oi = st.op0; oi < st.op0 + st.nops; oi++) {
+                    const BOpD &op = t.ops[oi];
+                    auto C = [&](uint32_t k) { return cd(t.coef[2 * (op.coef + k)], t.coef[2 * (op.coef + k) + 1]); };
+                    switch (op.kind) {
+                    case BK_U1:
+                    case BK_U1R:
+                    case BK_U1X:
+                        for (uint32_t j = 0; j < NL / 2; j++) {
+                            const uint32_t x0 = ins0h(j, op.p0), x1 = x0 | (1u << op.p0);
+                            const cd a = tl[x0], v = tl[x1];
+                            tl[x0] = C(0) * a + C(1) * v;
+                            tl[x1] = C(2) * a + C(3) * v;
+                        }
+                        break;
+                    case BK_U2:
+                        for (uint32_t j = 0; j < NL / 4; j++) {
+                            const uint32_t x = ins0h(ins0h(j, op.p0), op.p1);
+                            const uint32_t xs[4] = {x, x | (1u << op.p0), x | (1u << op.p1),
+                                                    x | (1u << op.p0) | (1u << op.p1)};
+                            cd v[4], r[4];
+                            for (int k = 0; k < 4; k++) v[k] = tl[xs[k]];
+                            for (int rr = 0; rr < 4; rr++) {
+                                r[rr] = 0;
+                                for (int cc = 0; cc < 4; cc++) r[rr] += C(4 * rr + cc) * v[cc];
+                            }
+                            for (int k = 0; k < 4; k++) tl[xs[k]] = r[k];
+                        }
+                        break;
+                    case BK_DM:
+                        if ((base & op.omask) != op.omask) break;
+                        for (uint32_t x = 0; x < NL; x++)
+                            if ((x & op.lmask) == op.lmask) tl[x] *= C(0);
+                        break;
+                    case BK_PERM1:
+                        if ((base & op.omask) != op.omask) break;
+                        for (uint32_t x0 = 0; x0 < NL; x0++) {
+                            if ((x0 >> op.p0) & 1 || (x0 & op.lmask) != op.lmask) continue;
+                            const uint32_t x1 = x0 | (1u << op.p0);
+                            const cd a = tl[x0], v = tl[x1];
+                            tl[x1] = ipow_c(op.k0, a);
+                            tl[x0] = ipow_c(op.k1, v);
+                        }
+                        break;
+                    case BK_SWAP:
+                        for (uint32_t x = 0; x < NL; x++)
+                            if (((x >> op.p0) & 1) && !((x >> op.p1) & 1))
+                                std::swap(tl[x], tl[x ^ (1u << op.p0) ^ (1u << op.p1)]);
+                        break;
+                    case BK_NSLOT: {
+                        const uint32_t code = ch.errs[en.err0 + op.a];
+                        if (!code) break;
+                        for (uint32_t j = 0; j < NL / 2; j++) {
+                            const uint32_t x0 = ins0h(j, op.p0), x1 = x0 | (1u << op.p0);
+                            cd a = tl[x0], v = tl[x1];
+                            if (code & 2) v = -v;
+                            if (code & 1) std::swap(a, v);
+                            tl[x0] = a;
+                            tl[x1] = v;
+                        }
+                        break;
+                    }
+                    case BK_MEASURE:
+                    case BK_REDUCE:
+                    case BK_COLLAPSE: {
+                        const uint32_t sgn = op.k0 ^ parity(base & op.omask);
+                        double nn = 0, e = 0;
+                        if (op.kind == BK_COLLAPSE) {
+                            for (uint32_t tt = 0; tt < op.b; tt++) {
+                                nn += part[en.part0 + op.a + 2 * tt];
+                                e += part[en.part0 + op.a + 2 * tt + 1];
+                            }
+                        } else {
+                            for (uint32_t x = 0; x < NL; x++) {
+                                const double w = std::norm(tl[x]);
+                                nn += w;
+                                e += (parity(x & op.lmask) ^ sgn) ? -w : w;
+                            }
+                        }
+                        if (op.kind == BK_REDUCE) {
+                            part[en.part0 + op.a + 2 * tileno] = nn;
+                            part[en.part0 + op.a + 2 * tileno + 1] = e;
+                            break;
+                        }
+                        const uint32_t ord = en.m0 + op.c;
+                        double p_plus = (1.0 + e / nn) / 2.0;
+                        if (!(p_plus >= 0.0)) p_plus = 0.0;
+                        if (p_plus > 1.0) p_plus = 1.0;
+                        uint32_t o;
+                        if (ch.forced[ord] != 0xff)
+                            o = ch.forced[ord] & 1;
+                        else if (p_plus > 1.0 - 1e-12)
+                            o = 0;
+                        else if (p_plus < 1e-12)
+                            o = 1;
+                        else
+                            o = ch.uni[ord] >= p_plus ? 1 : 0;
+                        const double kept = 0.5 * (nn + (o ? -e : e));
+                        double sc = 1.0;
+                        if (c64 && kept > 0.0 && kept < 0.25) sc = 1.0 / std::sqrt(kept);
+                        if (tileno == 0) {
+                            ch.outc[ord] = (uint8_t)o;
+                            if (sc != 1.0) ch.lsc[i] *= std::sqrt(kept);
+                        }
+                        if (op.p0 == 0xff) {
+                            for (uint32_t x = 0; x < NL; x++)
+                                tl[x] = ((parity(x & op.lmask) ^ sgn) != o) ? cd(0, 0) : tl[x] * sc;
+                        } else {
+                            for (uint32_t j = 0; j < NL / 2; j++) {
+                                const uint32_t x0 = ins0h(j, op.p0), x1 = x0 | (1u << op.p0);
+                                tl[x0] = (o ? tl[x1] : tl[x0]) * sc;
+                                tl[x1] = 0;
+                            }
+                        }
+                        break;
+                    }
+                    case BK_EXPECT: {
+                        for (uint32_t k = 0; k < op.b; k++) {
+                            const BPauliD &Pd = t.paulis[op.c + k];
+                            cd s(0, 0);
+                            for (uint32_t x = 0; x < NL; x++) {
+                                const uint32_t pw = (Pd.ph + 2 * (parity(Pd.zl & x) ^ parity(Pd.zo & base))) & 3;
+                                s += ipow_c(pw, std::conj(tl[x ^ Pd.xl]) * tl[x]);
+                            }
+                            ch.pexp[en.exp0 + op.a + 2ull * op.b * tileno + 2 * k] = s.real();
+                            ch.pexp[en.exp0 + op.a + 2ull * op.b * tileno + 2 * k + 1] = s.imag();
+                        }
+                        break;
+                    }
+                    default: fail(NCSV_INVALID, "batched emulator: unknown op");
+                    }
+                }
+                if (st.flags & BF_STORE || n == L)
+                    for (uint32_t x = 0; x < NL; x++) amp[addr[x]] = tl[x];
+            }
+        }
+    }
+}
+
+void collect(ncsv_batch *b, const Chunk &ch) {
+    for (size_t i = 0; i < ch.ent.size(); i++) {
+        const uint32_t e = ch.ent[i];
+        const auto &E = b->entries[e];
+        const Program &P = *b->progs[E.prog];
+        const BEntryD &en = ch.ed[i];
+        const double l2 = ch.lsc[i] * ch.lsc[i];
+        const size_t np = E.x.size();
+        std::vector<double> v(2 * (np + 1));
+        for (size_t k = 0; k <= np; k++) {
+            const ExpRef &r = P.exps[k];
+            // compensated sum over tiles (Neumaier)
+            for (int c = 0; c < 2; c++) {
+                double s = 0.0, comp = 0.0;
+                for (uint32_t t = 0; t < r.tiles; t++) {
+                    const double x = ch.pexp[en.exp0 + r.off + 2ull * r.K * t + 2 * r.k + c];
+                    const double y = s + x;
+                    comp += std::fabs(s) >= std::fabs(x) ? (s - y) + x : (x - y) + s;
+                    s = y;
+                }
+                v[2 * k + c] = (s + comp) * l2;
+            }
+        }
+        b->norm[e] = v[2 * np];
+        v.resize(2 * np);
+        b->expv[e] = std::move(v);
+        std::vector<uint8_t> oc(P.draw_kind.size(), 0xff);
+        for (size_t d = 0; d < oc.size(); d++)
+            if (P.draw_kind[d]) oc[d] = ch.outc[en.m0 + P.draw_ref[d]];
+        b->outc[e] = std::move(oc);
+    }
+}
+
+// mode 0: device, 1: host emulator, 2: the kernel body on the host
+void run_batch(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced, int mode) {
+    const bool emulate = mode != 0;
+    if (b->entries.empty()) return;
+    for (const auto &e : b->entries)
+        if (e.ndraws && !uniforms && !seeds) fail(NCSV_INVALID, "batched run: circuits with draws need uniforms or seeds");
+    plan_all(b);
+    const Tables t = flatten(b);
+    if (!emulate && b->tables_dirty) upload_tables(b, t);
+    std::vector<size_t> draw_off(b->entries.size() + 1, 0);
+    for (size_t e = 0; e < b->entries.size(); e++) draw_off[e + 1] = draw_off[e] + b->entries[e].ndraws;
+    b->expv.assign(b->entries.size(), {});
+    b->norm.assign(b->entries.size(), 0.0);
+    b->outc.assign(b->entries.size(), {});
+    b->kernel_ms = 0.0;
+    b->items = 0;
+    b->launches = 0;
+    b->amp_updates = 0.0;
+    b->hbm_bytes = 0.0;
+    const size_t es = b->prec == NCSV_C128 ? 16 : 8;
+    // chunks that fit the state pool
+    size_t limit = (size_t)64 << 30;
+    if (const char *e = getenv("NCSV_BATCH_MEM_GB")) limit = (size_t)(atof(e) * (double)(1ull << 30));
+    if (!emulate) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr) limit = std::min(limit, fr / 10 * 6);
+    }
+    std::vector<uint32_t> cur;
+    size_t cur_bytes = 0;
+    auto flush = [&]() {
+        if (cur.empty()) return;
+        Chunk ch = make_chunk(b, cur, uniforms, draw_off, seeds, forced);
+        if (mode == 1)
+            emulate_chunk(b, t, ch);
+        else if (mode == 2)
+            run_chunk_sim(b, t, ch);
+        else
+            run_chunk_device(b, ch);
+        collect(b, ch);
+        cur.clear();
+        cur_bytes = 0;
+    };
+    for (uint32_t e = 0; e < b->entries.size(); e++) {
+        const Program &P = *b->progs[b->entries[e].prog];
+        const size_t bytes = P.n > P.L ? (size_t(1) << P.n) * es : 0;
+        if (bytes > limit) fail(NCSV_DEVICE, "batched run: one state exceeds the memory limit");
+        if (cur_bytes + bytes > limit) flush();
+        cur.push_back(e);
+        cur_bytes += bytes;
+        b->amp_updates += (double)P.gates * (double)(1ull << P.n);
+        b->hbm_bytes += (double)P.hbm_elems * (double)es;
+    }
+    flush();
+}
+
+}  // namespace
+}  // namespace batch
+}  // namespace ncsv
+
+// ---------------------------------------------------------------- C ABI
+namespace ncsv {
+ncsv_status guard_status(const std::function<void()> &f);
+}
+
+extern "C" {
+
+ncsv_batch *ncsv_batch_new(ncsv_precision precision) {
+    ncsv_batch *out = nullptr;
+    const ncsv_status st = ncsv::guard_status([&]() {
+        if (precision != NCSV_C128 && precision != NCSV_C64) ncsv::fail(NCSV_INVALID, "unknown precision");
+        auto b = std::make_unique<ncsv_batch>();
+        b->prec = precision;
+        out = b.release();
+    });
+    return st == NCSV_OK ? out : nullptr;
+}
+
+void ncsv_batch_destroy(ncsv_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_amp) (void)hipFree(b->d_amp);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+ncsv_status ncsv_batch_add(ncsv_batch *b, const ncsv_circuit *c, uint32_t count, const uint64_t *x,
+                           const uint64_t *z, const uint32_t *phase, uint32_t *index) {
+    return ncsv::guard_status([&]() {
+        if (!b || !c) ncsv::fail(NCSV_INVALID, "null batch or circuit");
+        if (count && (!x || !z || !phase)) ncsv::fail(NCSV_INVALID, "null Pauli arrays");
+        if (c->n == 0 || c->n > 34) ncsv::fail(NCSV_INVALID, "batched form: 1..34 qubits per circuit");
+        const uint64_t full = c->n >= 64 ? ~0ull : (1ull << c->n) - 1;
+        for (uint32_t k = 0; k < count; k++)
+            if ((x[k] | z[k]) & ~full) ncsv::fail(NCSV_INVALID, "Pauli acts outside the circuit's qubits");
+        std::lock_guard<std::mutex> lk(b->mu);
+        ncsv_batch::Entry e;
+        e.c = c;
+        e.x.assign(x, x + count);
+        e.z.assign(z, z + count);
+        e.ph.assign(phase, phase + count);
+        e.ndraws = ncsv::batch::count_draws(c);
+        if (index) *index = (uint32_t)b->entries.size();
+        b->entries.push_back(std::move(e));
+    });
+}
+
+uint32_t ncsv_batch_size(const ncsv_batch *b) { return b ? (uint32_t)b->entries.size() : 0; }
+
+ncsv_status ncsv_batch_draws(const ncsv_batch *b, uint32_t entry, uint32_t *count) {
+    return ncsv::guard_status([&]() {
+        if (!b || !count || entry >= b->entries.size()) ncsv::fail(NCSV_INVALID, "bad batch entry");
+        *count = b->entries[entry].ndraws;
+    });
+}
+
+ncsv_status ncsv_batch_set_tile_bits(ncsv_batch *b, uint32_t bits) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        if (bits && (bits < 6 || bits > ncsv::batch::max_tile_bits(b->prec)))
+            ncsv::fail(NCSV_INVALID, "tile bits out of range");
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->tile_bits = bits;
+    });
+}
+
+ncsv_status ncsv_batch_set_blocks(ncsv_batch *b, int32_t blocks) {
+    return ncsv::guard_status([&]() {
+        if (!b || blocks < 0) ncsv::fail(NCSV_INVALID, "bad batch or block count");
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->blocks = blocks;
+    });
+}
+
+ncsv_status ncsv_batch_run(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        std::lock_guard<std::mutex> lk(b->mu);
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) ncsv::fail(NCSV_DEVICE, "no GPU device visible");
+        if (!b->stream) {
+            ncsv::check_hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+            ncsv::check_hip(hipEventCreate(&b->ev0), "hipEventCreate");
+            ncsv::check_hip(hipEventCreate(&b->ev1), "hipEventCreate");
+        }
+        ncsv::batch::run_batch(b, uniforms, seeds, forced, 0);
+    });
+}
+
+ncsv_status ncsv_batch_plan(ncsv_batch *b) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        std::lock_guard<std::mutex> lk(b->mu);
+        ncsv::batch::plan_all(b);
+    });
+}
+
+ncsv_status ncsv_batch_emulate(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        std::lock_guard<std::mutex> lk(b->mu);
+        ncsv::batch::run_batch(b, uniforms, seeds, forced, 1);
+    });
+}
+
+ncsv_status ncsv_batch_simulate(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        std::lock_guard<std::mutex> lk(b->mu);
+        ncsv::batch::run_batch(b, uniforms, seeds, forced, 2);
+    });
+}
+
+ncsv_status ncsv_batch_result(const ncsv_batch *b, uint32_t entry, double *expect, double *norm_sq,
+                              uint8_t *outcomes) {
+    return ncsv::guard_status([&]() {
+        if (!b || entry >= b->entries.size() || entry >= b->expv.size()) ncsv::fail(NCSV_INVALID, "bad batch entry");
+        const auto &v = b->expv[entry];
+        if (expect) std::copy(v.begin(), v.end(), expect);
+        if (norm_sq) *norm_sq = b->norm[entry];
+        if (outcomes) std::copy(b->outc[entry].begin(), b->outc[entry].end(), outcomes);
+    });
+}
+
+ncsv_status ncsv_batch_stats(const ncsv_batch *b, double *out) {
+    return ncsv::guard_status([&]() {
+        if (!b || !out) ncsv::fail(NCSV_INVALID, "null batch or output");
+        uint64_t steps = 0;
+        for (auto &P : b->progs) steps += P->steps.size();
+        out[0] = b->kernel_ms;
+        out[1] = b->plan_ms;
+        out[2] = (double)b->items;
+        out[3] = (double)b->launches;
+        out[4] = b->amp_updates;
+        out[5] = b->hbm_bytes;
+        out[6] = (double)b->progs.size();
+        out[7] = (double)steps;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// The batched form's persistent kernel for gfx950: workgroups claim (step,
+// tile) items of many entries from one queue, hold each tile in LDS while
+// the step's ops run on it, and order an entry's steps through completion
+// counters.  The kernel body is shared with the host simulation.
+#include "batch.hpp"
+#include "ncsv_internal.hpp"
+
+namespace ncsv {
+namespace batch {
+namespace kern {
+template <typename T> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+}  // namespace kern
+}  // namespace batch
+}  // namespace ncsv
+
+#define NCSV_DEV __device__
+#define NCSV_KERNEL __global__ __launch_bounds__(BT)
+#define NCSV_SHARED __shared__
+#define NCSV_TID threadIdx.x
+#define NCSV_SYNC() __syncthreads()
+#define NCSV_SHFL_XOR(x, m) __shfl_xor((x), (int)(m))
+#define NCSV_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#define NCSV_ATOMIC_OR(p, v) atomicOr((p), (v))
+#define NCSV_LOAD_AGENT(p) __hip_atomic_load((p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+#define NCSV_ADD_AGENT(p, v) __hip_atomic_fetch_add((p), (v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+#define NCSV_FENCE_ACQUIRE() __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent")
+#define NCSV_FENCE_RELEASE() __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent")
+#define NCSV_SLEEP() __builtin_amdgcn_s_sleep(8)
+#define NCSV_UNIFORM64(v) ::ncsv::batch::kern::uniform64(v)
+#define NCSV_UNIFORM32(v) __builtin_amdgcn_readfirstlane((uint32_t)(v))
+#define NCSV_LANE0_STORE(dst, v) ((dst) = ::ncsv::batch::kern::uniform64(v))
+
+namespace ncsv {
+namespace batch {
+namespace kern {
+__device__ inline uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+}  // namespace kern
+}  // namespace batch
+}  // namespace ncsv
+
+#include "batch_kernel.hpp"
+
+namespace ncsv {
+namespace batch {
+using kern::BT;
+using kern::k_batch;
+namespace {
+
+__global__ void k_batch_reset(uint32_t *done, uint32_t nrec, unsigned long long *next, uint32_t *flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrec) done[i] = 0;
+    if (i == 0) {
+        *next = 0;
+        *flag = 0;
+    }
+}
+
+}  // namespace
+
+uint32_t max_tile_bits(ncsv_precision prec) { return prec == NCSV_C128 ? 12u : 13u; }
+
+int resident_blocks(ncsv_precision prec) {
+    int dev = 0, cus = 0, per = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    if (prec == NCSV_C128)
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<double, 12>, BT, 0), "occupancy");
+    else
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<float, 13>, BT, 0), "occupancy");
+    return cus * std::max(per, 1);
+}
+
+void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_reset, dim3((nrec + 255) / 256), dim3(256), 0, stream, A.done, nrec, A.next, A.flag);
+    check_hip(hipGetLastError(), "k_batch_reset");
+    if (prec == NCSV_C128)
+        hipLaunchKernelGGL((k_batch<double, 12>), dim3(blocks), dim3(BT), 0, stream, A);
+    else
+        hipLaunchKernelGGL((k_batch<float, 13>), dim3(blocks), dim3(BT), 0, stream, A);
+    check_hip(hipGetLastError(), "k_batch");
+}
+
+}  // namespace batch
+}  // namespace ncsv

@@ -1,0 +1,109 @@
+// Batched form: many circuits or noise trajectories of uneven sizes run by
+// one persistent kernel launch (no counterpart in the reference, which runs
+// one state at a time: mod.rs:168-183 per circuit).
+//
+// Each entry (a circuit started from |0...0>, with Paulis to evaluate at its
+// end) follows a program of steps.  A step is one pass over the entry's
+// state in tiles of 2^L amplitudes (the amplitudes that agree on the n-L
+// outer qubits), each tile held in LDS while the step's ops run on it:
+//   gate ops   — the fusion planner's tile ops, on tile positions,
+//   NSLOT      — the Pauli error drawn for a noise site of this trajectory,
+//   MEASURE    — whole-state tile: reduce, decide, collapse, reset,
+//   REDUCE     — per-tile FP64 partials (norm, <Z-string>) for a measurement,
+//   COLLAPSE   — sums a REDUCE's partials (every tile the same way), decides
+//                the outcome, collapses the tile, optional reset,
+//   EXPECT     — per-tile FP64 partials of up to 9 Pauli expectations.
+// An entry of n <= L qubits is one step on one tile: its state never leaves
+// LDS.  Larger entries keep their state in HBM between steps.
+//
+// The launch claims (step, tile) items from one counter in depth-major
+// order (step d of every entry before step d+1 of any), and an item waits
+// for all tiles of its entry's previous step through that step's completion
+// counter; the lowest unfinished item never waits, so the queue drains.
+#pragma once
+
+#include <cstdint>
+
+namespace ncsv {
+namespace batch {
+
+enum BOpKind : uint8_t {
+    BK_U1 = 0,      // 2x2 on position p0 (coef: 4)
+    BK_U1R = 1,     // 2x2 with real entries
+    BK_U1X = 2,     // real diagonal, imaginary off-diagonal
+    BK_U2 = 3,      // 4x4 on positions p0 < p1 (coef: 16)
+    BK_DM = 4,      // phase coef[0] where (x & lmask) == lmask and outer bits omask set
+    BK_PERM1 = 5,   // flip p0 where the conditions hold: new1 = i^k0 v0, new0 = i^k1 v1
+    BK_SWAP = 6,    // swap positions p0 < p1
+    BK_NSLOT = 7,   // noise slot a on position p0: error code bit0 = X, bit1 = Z (Z first)
+    BK_MEASURE = 8, // whole tile: measure Z-string (lmask, sign k0), ordinal a, reset p0 (0xff none)
+    BK_REDUCE = 9,  // partials (norm, <Z-string>) to part[a + 2 tile]; lmask/omask = z, k0 = sign
+    BK_COLLAPSE = 10,  // from b tiles of partials at a; ordinal c; lmask/omask = z, k0 = sign; reset p0
+    BK_EXPECT = 11, // Paulis [c, c+b) to part[a + 2b tile + 2k]
+};
+
+struct BOpD {
+    uint8_t kind, p0, p1, k0, k1, pad0, pad1, pad2;
+    uint32_t lmask;  // condition / Z mask on tile positions
+    uint32_t a, b, c;
+    uint32_t coef;   // first complex coefficient
+    uint64_t omask;  // condition / Z mask on outer qubits (global qubit bits)
+};
+
+struct BPauliD {
+    uint32_t xl, zl;  // on tile positions
+    uint64_t zo;      // on outer qubits
+    uint32_t ph, pad;
+};
+
+enum BStepFlags : uint32_t { BF_INIT = 1, BF_LOAD = 2, BF_STORE = 4 };
+
+struct BStepD {
+    uint32_t L, n;
+    uint32_t op0, nops;
+    uint32_t flags;
+    uint32_t qoff;  // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
+    uint32_t pad[2];
+};
+
+struct BEntryD {
+    uint64_t amp_off;  // first amplitude in the pool (multi-tile entries)
+    uint64_t part0;    // first measurement partial (doubles)
+    uint64_t exp0;     // first expectation partial (doubles)
+    uint32_t m0;       // first measurement ordinal slot (uniforms, outcomes, forced)
+    uint32_t err0;     // first noise slot
+};
+
+struct BRecD {
+    uint64_t first;       // first item number of this (entry, step)
+    uint32_t entry, step;  // step: index into the step table
+    uint32_t prev;        // record of the entry's previous step, or ~0u
+    uint32_t prev_tiles;  // its tile count
+};
+
+struct BArgs {
+    const BStepD *steps;
+    const BOpD *ops;
+    const double *coef;  // complex coefficients as (re, im)
+    const BPauliD *paulis;
+    const uint8_t *qtab;
+    const BEntryD *entries;
+    const BRecD *recs;        // + sentinel
+    const uint32_t *rec_blk;  // record holding item 16k
+    uint64_t total;           // items
+    void *amp;
+    double *part;             // measurement partials
+    double *pexp;             // expectation partials
+    double *lsc;              // per entry: logical scale (complex64 renormalization)
+    const double *uni;        // per measurement ordinal
+    const uint8_t *forced;    // per ordinal: 0, 1 or 0xff (free); may be null
+    uint8_t *outc;            // per ordinal
+    const uint8_t *errs;      // per noise slot
+    uint32_t *done;           // per record
+    unsigned long long *next;  // queue counter
+    uint32_t *flag;           // bit 0: a wait ran out of spins
+    uint32_t spin_limit;
+};
+
+}  // namespace batch
+}  // namespace ncsv

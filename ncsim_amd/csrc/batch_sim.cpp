@@ -1,0 +1,110 @@
+// Host build of the batched form's kernel body (batch_kernel.hpp): one
+// workgroup runs as BT host threads, with a real barrier for each
+// workgroup barrier and a shared exchange for the wavefront shuffle, so the
+// kernel's own logic (queue, dependencies, ops, reductions) can be run and
+// checked without a device.  One workgroup claims every item in order, so
+// every dependency is met when it is reached.
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "batch.hpp"
+#include "ncsv_internal.hpp"
+
+namespace ncsv {
+namespace batch {
+namespace kern {
+struct hf2 {
+    float x, y;
+};
+struct hd2 {
+    double x, y;
+};
+template <typename T> struct V2;
+template <> struct V2<float> { using type = hf2; };
+template <> struct V2<double> { using type = hd2; };
+}  // namespace kern
+
+namespace sim {
+struct Barrier {
+    std::mutex m;
+    std::condition_variable cv;
+    unsigned n, count = 0, gen = 0;
+    explicit Barrier(unsigned n_) : n(n_) {}
+    void wait() {
+        std::unique_lock<std::mutex> lk(m);
+        const unsigned g = gen;
+        if (++count == n) {
+            count = 0;
+            gen++;
+            cv.notify_all();
+            return;
+        }
+        cv.wait(lk, [&] { return gen != g; });
+    }
+};
+thread_local unsigned tid = 0;
+Barrier *bar = nullptr;
+double xbuf[1024];
+double shfl_xor(double x, unsigned mask) {
+    xbuf[tid] = x;
+    bar->wait();
+    const double r = xbuf[tid ^ mask];
+    bar->wait();
+    return r;
+}
+}  // namespace sim
+}  // namespace batch
+}  // namespace ncsv
+
+#define NCSV_DEV
+#define NCSV_KERNEL
+#define NCSV_SHARED static
+#define NCSV_TID (::ncsv::batch::sim::tid)
+#define NCSV_SYNC() ::ncsv::batch::sim::bar->wait()
+#define NCSV_SHFL_XOR(x, m) ::ncsv::batch::sim::shfl_xor((x), (m))
+#define NCSV_ATOMIC_ADD(p, v) __atomic_fetch_add((p), (v), __ATOMIC_SEQ_CST)
+#define NCSV_ATOMIC_OR(p, v) __atomic_fetch_or((p), (v), __ATOMIC_SEQ_CST)
+#define NCSV_LOAD_AGENT(p) __atomic_load_n((p), __ATOMIC_SEQ_CST)
+#define NCSV_ADD_AGENT(p, v) __atomic_fetch_add((p), (v), __ATOMIC_SEQ_CST)
+#define NCSV_FENCE_ACQUIRE() __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#define NCSV_FENCE_RELEASE() __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#define NCSV_SLEEP() std::this_thread::yield()
+#define NCSV_UNIFORM64(v) (v)
+#define NCSV_UNIFORM32(v) (v)
+#define NCSV_LANE0_STORE(dst, v) \
+    do {                         \
+        if (NCSV_TID == 0) (dst) = (v); \
+    } while (0)
+
+#include "batch_kernel.hpp"
+
+namespace ncsv {
+namespace batch {
+
+// Runs the kernel body over the whole queue with one simulated workgroup.
+void simulate(const BArgs &A, ncsv_precision prec) {
+    static std::mutex one;  // the simulated workgroup's LDS is static
+    std::lock_guard<std::mutex> lk(one);
+    sim::Barrier b(kern::BT);
+    sim::bar = &b;
+    std::vector<std::thread> th;
+    th.reserve(kern::BT);
+    for (unsigned t = 0; t < kern::BT; t++) {
+        th.emplace_back([&A, prec, t]() {
+            sim::tid = t;
+            if (prec == NCSV_C128)
+                kern::k_batch<double, 12>(A);
+            else
+                kern::k_batch<float, 13>(A);
+        });
+    }
+    for (auto &x : th) x.join();
+    sim::bar = nullptr;
+}
+
+}  // namespace batch
+}  // namespace ncsv

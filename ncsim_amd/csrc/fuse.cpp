@@ -30,6 +30,7 @@ struct Cls {
     uint64_t cmask = 0;  // DM: qubits that must all be 1; PERM1: controls
     cd m[16];
     uint8_t k0 = 0, k1 = 0;
+    uint32_t slot = 0;  // NSLOT
 };
 
 int quarter(cd c) {
@@ -80,6 +81,13 @@ Cls classify(const LOp &op) {
         }
         return c;
     }
+    case OpKind::Slot:
+        c.tileable = true;
+        c.kind = TK_NSLOT;
+        c.q0 = op.qubits[0];
+        c.act = c.sup;
+        c.slot = op.ph;
+        return c;
     case OpKind::Perm: {
         const uint32_t d = 1u << k;
         bool all_one = true;
@@ -304,6 +312,7 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
     uint32_t top = L - R;
     for (uint64_t a = st.empty() ? 0 : st[0].first; a; a &= a - 1) {
         const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        while (top < L && sw.tq[top] != ~0u) top++;  // R = L: skip the fixed low positions
         if (pos_of[q] != ~0u || top >= L) continue;
         sw.tq[top] = q;
         pos_of[q] = top++;
@@ -412,6 +421,10 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
                 o.k0 = g.k0;
                 o.k1 = g.k1;
                 break;
+            case TK_NSLOT:
+                o.j0 = (uint8_t)reg_bit(g.q0, pos_of.data(), rpos);
+                o.aux = g.slot;
+                break;
             default: fail(NCSV_INVALID, "tile planner: unknown op");
             }
             for (uint64_t a = g.sup; a; a &= a - 1) last[__builtin_ctzll(a)] = (int)sw.ops.size();
@@ -427,13 +440,13 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
 }
 
 std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t> rem, uint32_t n,
-                                const TileGeom &geo) {
+                                const TileGeom &geo, uint64_t first_fixed = 0) {
     std::vector<SweepH> out;
     const uint32_t L = geo.L();
     const uint64_t fixed = (1ull << std::min(geo.c, n)) - 1;
     const uint64_t pool = n >= 64 ? ~0ull : (1ull << n) - 1;
     while (!rem.empty()) {
-        const uint64_t T = choose_set(cl, rem, fixed, pool, L);
+        const uint64_t T = choose_set(cl, rem, fixed | (out.empty() ? first_fixed : 0), pool, L);
         std::vector<uint32_t> got;
         pick(cl, rem, T, &got, nullptr);
         if (got.empty()) fail(NCSV_INVALID, "sweep planner made no progress");
@@ -468,6 +481,37 @@ struct Plan {
 
 
 }  // namespace
+
+std::vector<SweepH> plan_lds_sweeps(const std::vector<LOp> &ops, uint32_t n, uint32_t L, uint32_t c,
+                                    uint64_t first_fixed) {
+    if (L > n || c > L || (uint32_t)__builtin_popcountll(first_fixed | ((1ull << c) - 1)) > L)
+        fail(NCSV_INVALID, "tile planner: bad tile shape");
+    std::vector<Cls> cl;
+    std::vector<uint32_t> run;
+    for (const LOp &op : ops) {
+        if (op.kind == OpKind::Nop) continue;
+        Cls k = classify(op);
+        if (!k.tileable) fail(NCSV_UNSUPPORTED, "batched form: a gate on " + std::to_string(op.k()) +
+                                                     " qubits that is not a tile op (dense on more than 2 qubits)");
+        cl.push_back(k);
+        run.push_back((uint32_t)cl.size() - 1);
+    }
+    TileGeom geo;
+    geo.LT = 0;
+    geo.R = L;
+    geo.c = c;
+    std::vector<SweepH> out = plan_sweeps(cl, run, n, geo, first_fixed);
+    if (out.empty() && first_fixed) {
+        // no gates: an empty sweep whose tile holds the fixed qubits
+        SweepH sw;
+        uint64_t T = ((1ull << c) - 1) | first_fixed;
+        for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(T) < L; q++) T |= 1ull << q;
+        for (uint32_t q = 0; q < c; q++) sw.tq.push_back(q);
+        for (uint64_t a = T & ~((1ull << c) - 1); a; a &= a - 1) sw.tq.push_back((uint32_t)__builtin_ctzll(a));
+        out.push_back(sw);
+    }
+    return out;
+}
 
 bool use_specialized(const ncsv_state *s) {
     if (!jit::available() || s->fusion == 0) return false;

@@ -28,12 +28,14 @@ enum TileKind : uint8_t {
     TK_SWAPR = 6,  // swap register bits j0 < j1
     TK_U1R = 7,    // 2x2 with real entries (H, RY, ...)
     TK_U1X = 8,    // 2x2 with a real diagonal and an imaginary off-diagonal (RX, ...)
+    TK_NSLOT = 9,  // a noise site on register bit j0: the Pauli error drawn for slot `aux` (batched form)
 };
 
 struct TileOpH {
     uint8_t kind = 0, j0 = 0, j1 = 0, k0 = 0, k1 = 0;
     uint32_t rmask = 0;
     uint64_t gmask = 0;
+    uint32_t aux = 0;
     cd m[16];
 };
 
@@ -64,6 +66,13 @@ struct TileGeom {
 
 // The fused schedule of a segment of lowered gates on n qubits.
 std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const TileGeom &geo);
+
+// Tile sweeps for a run of lowered gates when every tile position is
+// addressable (the batched form's LDS-resident tiles): L tile qubits of which
+// the low c are qubits 0..c-1; `first_fixed` qubits must be tile qubits of
+// the first sweep.  Fails with NCSV_UNSUPPORTED on gates no tile op covers.
+std::vector<SweepH> plan_lds_sweeps(const std::vector<LOp> &ops, uint32_t n, uint32_t L, uint32_t c,
+                                    uint64_t first_fixed);
 
 // Whether the state runs kernels specialized per plan (fusion 2, or fusion 1
 // from NCSV_JIT_MIN_QUBITS = 20 qubits on; never with NCSV_JIT=0).

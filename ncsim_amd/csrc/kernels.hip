@@ -722,6 +722,7 @@ template <typename T> static void apply_lop_t(ncsv_state *s, const LOp &op) {
     switch (op.kind) {
     case OpKind::Nop: return;
     case OpKind::Pauli: return;
+    case OpKind::Slot: fail(NCSV_INVALID, "noise slots exist only in batched programs");
     case OpKind::Diag: {
         if (k > 4) fail(NCSV_UNSUPPORTED, "diagonal op on more than 4 qubits");
         for (int x = 0; x < (1 << k); x++) {

@@ -51,7 +51,8 @@ cd sqrt2_pow(int64_t k);
 
 // A gate lowered to what the dense kernels understand, acting on `qubits`
 // of the register (local bit j = qubits[j]).
-enum class OpKind { Dense, Diag, Perm, Pauli, Nop };
+// Slot: a noise site of the batched form on qubits[0] (slot index in `ph`).
+enum class OpKind { Dense, Diag, Perm, Pauli, Nop, Slot };
 struct LOp {
     OpKind kind = OpKind::Nop;
     std::vector<uint32_t> qubits;

@@ -108,6 +108,64 @@ def qft_with_rz(n, rz_count, seed, prefix_x=True):
     return "\n".join(out + body) + "\n"
 
 
+def batch_circuits(count, seed, qmin=10, qmax=22, dmin=10, dmax=120, npauli=8):
+    """Uneven small circuits: qubits uniform in [qmin, qmax], depth uniform in
+    [dmin, dmax]; per layer every qubit gets one of H, S, T, SQRT_X or
+    RZ(theta ~ U[0, 2pi)), then CX on a random perfect matching with each pair
+    kept with probability 1/2, then with probability 0.05 a CCZ on a random
+    triple.  Each comes with `npauli` Pauli strings of weight 1-4.
+    Returns [(n, text, [(x, z, phase), ...])]."""
+    r = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = qmin + r.below(qmax - qmin + 1)
+        depth = dmin + r.below(dmax - dmin + 1)
+        lines = [f"QUBITS {n}"]
+        for _ in range(depth):
+            for q in range(n):
+                k = r.below(5)
+                if k == 4:
+                    lines.append(f"RZ({r.gen_f64() * 2.0 * math.pi!r}) {q}")
+                else:
+                    lines.append(f"{['H', 'S', 'T', 'SQRT_X'][k]} {q}")
+            perm = r.shuffle(list(range(n)))
+            for i in range(0, n - 1, 2):
+                if r.gen_f64() < 0.5:
+                    lines.append(f"CX {perm[i]} {perm[i + 1]}")
+            if r.gen_f64() < 0.05:
+                a, b, c = r.shuffle(list(range(n)))[:3]
+                lines.append(f"CCZ {a} {b} {c}")
+        out.append((n, "\n".join(lines) + "\n", random_paulis(n, npauli, r.next_u64())))
+    return out
+
+
+def noisy_brickwork(n, depth, seed, p=1e-3, every=10, mqubits=None):
+    """Noisy trajectories' circuit: per layer a Haar-random SU(2) on every
+    qubit as RZ(gamma) RY(beta) RZ(alpha) (alpha, gamma ~ U[0, 2pi), beta =
+    arccos(1 - 2u)) followed by DEPOLARIZE1(p), then CX on alternating pairs
+    ((2i, 2i+1) in even layers, (2i+1, 2i+2) in odd ones) each followed by
+    DEPOLARIZE1(p) on both qubits; after every `every` layers M then R on four
+    fixed qubits."""
+    r = SplitMix64(seed)
+    mq = mqubits if mqubits is not None else [0, n // 4, n // 2, (3 * n) // 4]
+    lines = [f"QUBITS {n}"]
+    for layer in range(depth):
+        for q in range(n):
+            a, u, g = r.gen_f64() * 2.0 * math.pi, r.gen_f64(), r.gen_f64() * 2.0 * math.pi
+            b = math.acos(1.0 - 2.0 * u)
+            lines.append(f"RZ({g!r}) {q}")
+            lines.append(f"RY({b!r}) {q}")
+            lines.append(f"RZ({a!r}) {q}")
+            lines.append(f"DEPOLARIZE1({p!r}) {q}")
+        for a in range(layer % 2, n - 1, 2):
+            lines.append(f"CX {a} {a + 1}")
+            lines.append(f"DEPOLARIZE1({p!r}) {a} {a + 1}")
+        if (layer + 1) % every == 0:
+            lines.append("M " + " ".join(str(q) for q in mq))
+            lines.append("R " + " ".join(str(q) for q in mq))
+    return "\n".join(lines) + "\n"
+
+
 def count_gates(text):
     """Unfused gate applications of a text circuit (targets grouped by arity)."""
     ar = {"CX": 2, "CY": 2, "CZ": 2, "SWAP": 2, "CS": 2, "CS_DAG": 2, "CPHASE": 2, "CNOT": 2, "CCZ": 3, "CCX": 3,

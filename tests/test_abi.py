@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(ROOT, "include", "ncsv.h")).read()
-    return sorted(set(re.findall(r"^(?:ncsv_status|size_t|int)\s+(ncsv_[a-z0-9_]+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:ncsv_status|size_t|int|void|uint32_t|ncsv_batch \*)\s*(ncsv_[a-z0-9_]+)\(",
+                                 text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,297 @@
+"""The batched form (many circuits or trajectories per launch) against the
+oracle.  Each entry is the reference's run (mod.rs:168-183) from |0...0>
+with one uniform per Measure/Reset/Noise op in program order, followed by
+StateVector::expectation (vector.rs:412-429) of its Paulis.
+
+CPU tests run the batch programs through the library's FP64 host emulator
+(the same step/op lists the kernel runs), so the planner's tiles, reduce /
+collapse steps and noise slots are checked without a device; GPU tests run
+the persistent kernel."""
+import math
+
+import numpy as np
+import pytest
+
+import ncsim_amd as nc
+from oracle import circuit as OC
+from oracle import statevector as OS
+from oracle.pauli import Pauli as OPauli
+
+ONE_Q = ["H", "S", "T", "SQRT_X", "X", "Y", "Z", "S_DAG", "SQRT_Y"]
+TWO_Q = ["CX", "CZ", "SWAP", "CY", "CS"]
+
+
+def random_circuit(n, depth, seed, measure=True, noise=True):
+    """Random circuit text: rotations, Clifford+T, CCZ/CCX, mid-circuit
+    measurements (M, MPP of Z strings), resets and Pauli noise."""
+    rng = np.random.default_rng(seed)
+    out = [f"QUBITS {n}"]
+    for _ in range(depth):
+        for q in range(n):
+            r = rng.random()
+            if r < 0.25:
+                out.append(f"{['RX', 'RY', 'RZ'][rng.integers(3)]}({rng.uniform(0, 2 * math.pi)!r}) {q}")
+            else:
+                out.append(f"{ONE_Q[rng.integers(len(ONE_Q))]} {q}")
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            if rng.random() < 0.6:
+                out.append(f"{TWO_Q[rng.integers(len(TWO_Q))]} {perm[i]} {perm[i + 1]}")
+        if n >= 3 and rng.random() < 0.3:
+            a, b, c = rng.choice(n, 3, replace=False)
+            out.append(f"{'CCZ' if rng.random() < 0.5 else 'CCX'} {a} {b} {c}")
+        if noise and rng.random() < 0.5:
+            q = int(rng.integers(n))
+            out.append(f"DEPOLARIZE1(0.3) {q}")
+            if rng.random() < 0.3:
+                a, b = rng.choice(n, 2, replace=False)
+                out.append(f"DEPOLARIZE2(0.2) {a} {b}")
+        if measure and rng.random() < 0.3:
+            r = rng.random()
+            q = int(rng.integers(n))
+            if r < 0.4:
+                out.append(f"M {q}")
+            elif r < 0.7:
+                out.append(f"R {q}")
+            else:
+                a, b = rng.choice(n, 2, replace=False)
+                out.append(f"MPP Z{a}*Z{b}")
+    return "\n".join(out) + "\n"
+
+
+def random_paulis(n, count, seed):
+    rng = np.random.default_rng(seed + 7)
+    ps = []
+    for _ in range(count):
+        w = int(rng.integers(1, min(4, n) + 1))
+        sup = rng.choice(n, w, replace=False)
+        x = z = 0
+        for q in sup:
+            k = int(rng.integers(1, 4))
+            if k in (1, 2):
+                x |= 1 << int(q)
+            if k in (2, 3):
+                z |= 1 << int(q)
+        ps.append(OPauli(n, x, z, int(rng.integers(4))))
+    return ps
+
+
+class _One:
+    def __init__(self, u):
+        self.u = u
+
+    def gen_f64(self):
+        return self.u
+
+
+def oracle_entry(text, paulis, us, forced=None):
+    """The oracle run with the batched form's draw rule."""
+    c = OC.Circuit.parse(text)
+    sv = OS.StateVector.zero_state(c.n)
+    k = 0
+    outc = []
+    for op in c.ops:
+        if op[0] not in ("measure", "reset", "noise"):
+            sv.apply_op(op, None)
+            continue
+        u = us[k]
+        f = None if forced is None else forced[k]
+        k += 1
+        if op[0] == "noise":
+            sv.apply_op(op, _One(u))
+            outc.append(None)
+            continue
+        p = op[1] if op[0] == "measure" else OPauli.single(c.n, op[1], "Z")
+        if f is not None and f != 0xff:
+            sv.project(p, bool(f))
+            b = bool(f)
+        else:
+            b = sv.measure(p, _One(u))
+        if op[0] == "reset" and b:
+            from oracle.gates import Gate
+            sv.apply_gate(Gate("X"), [op[1]])
+        outc.append(int(b))
+    ex = [sv.expectation(p) for p in paulis]
+    return ex, sv.norm_sq().real, outc
+
+
+def _build(texts, paulis, precision, tile_bits):
+    b = nc.HipBatch(precision)
+    if tile_bits:
+        b.set_tile_bits(tile_bits)
+    circs = [nc.Circuit.parse(t) for t in texts]
+    for c, ps in zip(circs, paulis):
+        b.add(c, ps)
+    return b
+
+
+def _draws(b, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.random(b.draws(i)) for i in range(len(b))]
+
+
+CASES = [(6, 8, 1), (7, 10, 2), (9, 6, 3), (10, 9, 4), (11, 5, 5), (12, 4, 6)]
+
+
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+@pytest.mark.parametrize("tile_bits", [0, 6, 8])
+def test_emulated_batch_matches_oracle(precision, tile_bits):
+    """Whole-state tiles (tile_bits 0) and multi-tile programs (6, 8 tile
+    qubits), with measurements, resets and noise, FP64 host emulation."""
+    texts = [random_circuit(n, d, s) for n, d, s in CASES]
+    paulis = [random_paulis(n, 5, s) for n, d, s in CASES]
+    b = _build(texts, paulis, precision, tile_bits)
+    us = _draws(b, 11)
+    b.emulate(uniforms=us)
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc, f"entry {i}"
+        assert abs(nn - rnn) < 1e-12
+        for a, r in zip(ex, rex):
+            assert abs(a - r) < 1e-12, (i, a, r)
+
+
+def test_emulated_batch_seeds_and_forced():
+    """Seeded draws equal the SplitMix64 stream; forced outcomes override."""
+    n, d, s = 9, 8, 21
+    text = random_circuit(n, d, s)
+    ps = random_paulis(n, 3, s)
+    b = _build([text, text], [ps, ps], nc.C128, 7)
+    b.emulate(seeds=[5, 6])
+    for i, seed in enumerate([5, 6]):
+        g = OS.SplitMix64(seed)
+        us = [g.gen_f64() for _ in range(b.draws(i))]
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(text, ps, us)
+        assert oc == roc
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+    # forced outcomes: other uniforms at the measurement slots, the outcomes
+    # of the first run forced, give the first run's results again
+    us = _draws(b, 3)
+    b.emulate(uniforms=us)
+    first = [b.result(i) for i in range(2)]
+    rng = np.random.default_rng(8)
+    us2, forced = [], []
+    for i in range(2):
+        oc = first[i][2]
+        us2.append(np.array([u if o is None else rng.random() for u, o in zip(us[i], oc)]))
+        forced.append(np.array([0xff if o is None else o for o in oc], dtype=np.uint8))
+    assert any(o is not None for o in first[0][2])
+    b.emulate(uniforms=us2, forced=forced)
+    for i in range(2):
+        ex, nn, oc = b.result(i)
+        assert oc == first[i][2]
+        assert ex == first[i][0] and nn == first[i][1]
+
+
+def test_batch_programs_share_and_count():
+    """Entries running one circuit share its program; stats count them."""
+    text = random_circuit(8, 5, 9, measure=True, noise=True)
+    ps = random_paulis(8, 2, 9)
+    b = nc.HipBatch(nc.C64)
+    b.set_tile_bits(6)
+    c = nc.Circuit.parse(text)
+    for _ in range(5):
+        b.add(c, ps)
+    b.emulate(seeds=list(range(5)))
+    st = b.stats()
+    assert st["programs"] == 1
+    assert st["amp_updates"] > 0
+    for i in range(5):
+        g = OS.SplitMix64(i)
+        us = [g.gen_f64() for _ in range(b.draws(i))]
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(text, ps, us)
+        assert oc == roc
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+def test_kernel_code_on_host_matches_oracle(precision):
+    """The device kernel's own source run on the host (one workgroup as 512
+    threads with real barriers): whole-state entries and a multi-tile entry
+    whose steps wait on each other."""
+    cases = [(5, 4, 31), (8, 5, 32), (9, 4, 33)]
+    texts = [random_circuit(n, d, s) for n, d, s in cases]
+    paulis = [random_paulis(n, 3, s) for n, d, s in cases]
+    b = _build(texts, paulis, precision, 7)
+    us = _draws(b, 21)
+    b.simulate(uniforms=us)
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc, f"entry {i}"
+        assert abs(nn - rnn) < tol
+        for a, r in zip(ex, rex):
+            assert abs(a - r) < tol, (i, a, r)
+
+
+def test_batch_rejects_unsupported():
+    b = nc.HipBatch(nc.C128)
+    c = nc.Circuit.parse("QUBITS 3\nH 0\nMX 1\n")
+    b.add(c, [])
+    with pytest.raises(nc.NcsvError):
+        b.emulate(uniforms=[np.array([0.5])])
+
+
+# ----------------------------------------------------------------- GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+@pytest.mark.parametrize("tile_bits", [0, 7])
+def test_gpu_batch_matches_oracle(precision, tile_bits):
+    texts = [random_circuit(n, d, s) for n, d, s in CASES]
+    paulis = [random_paulis(n, 5, s) for n, d, s in CASES]
+    b = _build(texts, paulis, precision, tile_bits)
+    us = _draws(b, 13)
+    b.run(uniforms=us)
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc, f"entry {i}"
+        assert abs(nn - rnn) < tol * max(1.0, rnn)
+        for a, r in zip(ex, rex):
+            assert abs(a - r) < tol, (i, a, r)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_uneven_sizes_and_many_entries():
+    """A few hundred entries of 4..16 qubits (whole-state and HBM-resident)
+    in one launch, each equal to its own oracle run."""
+    rng = np.random.default_rng(5)
+    specs = [(int(rng.integers(4, 17)), int(rng.integers(2, 12)), 1000 + i) for i in range(160)]
+    texts = [random_circuit(n, d, s, measure=True, noise=True) for n, d, s in specs]
+    paulis = [random_paulis(n, 3, s) for n, d, s in specs]
+    b = _build(texts, paulis, nc.C128, 0)
+    us = _draws(b, 17)
+    b.run(uniforms=us)
+    for i in range(0, len(specs), 7):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(texts[i], paulis[i], us[i])
+        assert oc == roc, f"entry {i}"
+        assert max([0.0] + [abs(a - r) for a, r in zip(ex, rex)]) < 1e-12
+        assert abs(nn - rnn) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_batch_equals_emulator_and_single_state():
+    """The kernel equals the host emulator; a noiseless unitary entry equals
+    the single-state fused run of the same circuit."""
+    texts = [random_circuit(14, 12, 40, measure=False, noise=False), random_circuit(16, 10, 41, measure=True)]
+    paulis = [random_paulis(14, 8, 40), random_paulis(16, 8, 41)]
+    b = _build(texts, paulis, nc.C128, 0)
+    us = _draws(b, 19)
+    b.run(uniforms=us)
+    dev = [b.result(i) for i in range(2)]
+    b.emulate(uniforms=us)
+    emu = [b.result(i) for i in range(2)]
+    for (e1, n1, o1), (e2, n2, o2) in zip(dev, emu):
+        assert o1 == o2
+        assert abs(n1 - n2) < 1e-12
+        assert max(abs(a - c) for a, c in zip(e1, e2)) < 1e-12
+    s = nc.HipStateVector.zero_state(14)
+    s.run(nc.Circuit.parse(texts[0]))
+    ref = s.expectations([nc.Pauli(14, p.x, p.z, p.phase) for p in paulis[0]])
+    assert max(abs(a - c) for a, c in zip(dev[0][0], ref)) < 1e-12
