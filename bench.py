@@ -118,8 +118,9 @@ def run_batch(args):
     10-120, H/S/T/SQRT_X/RZ layers, CX matchings, CCZ with p = 0.05) with 8
     Paulis each, all in one persistent launch per run.  Compared with
     per-circuit launches (each circuit on its own state through the fused
-    single-state path) over every `--stride`-th circuit, scaled to the batch.
-    value = circuits/s of the batched form."""
+    single-state path) over every `--stride`-th circuit; the same for the
+    subset of 10-14 qubits.  value = circuits/s of the batched form on the
+    whole set."""
     import numpy as np
 
     import ncsim_amd as H
@@ -131,52 +132,64 @@ def run_batch(args):
     paulis = [[H.Pauli(n, x, z, ph) for x, z, ph in ps] for n, _, ps in specs]
     gates = np.array([W.count_gates(t) for _, t, _ in specs], dtype=np.float64)
     ns = np.array([n for n, _, _ in specs])
+
+    def batched(idx):
+        b = H.HipBatch(prec)
+        if args.tile_bits:
+            b.set_tile_bits(args.tile_bits)
+        for i in idx:
+            b.add(circs[i], paulis[i])
+        t0 = time.perf_counter()
+        b.run()  # plans every circuit and uploads the programs
+        cold = time.perf_counter() - t0
+        for _ in range(max(0, args.warmup - 1)):
+            b.run()
+        t0 = time.perf_counter()
+        kms = 0.0
+        for _ in range(args.steps):
+            b.run()
+            kms += b.stats()["kernel_ms"]
+        wall = (time.perf_counter() - t0) / args.steps
+        st = b.stats()
+        log(f"batch of {len(idx)}: {wall * 1000:.1f} ms per run (kernel {kms / args.steps:.1f} ms, first run "
+            f"{cold:.2f} s, planning {st['plan_ms']:.0f} ms), {st['steps']:.0f} steps, {st['tiles']:.0f} tiles")
+        return b, wall, kms / args.steps, cold
+
+    os.environ["NCSV_JIT_MIN_QUBITS"] = "99"  # per-circuit runs: no per-circuit kernel compiles
+
+    def per_circuit(idx, b, pos):
+        states = {}
+        for i in idx:  # warm-up: plans cached on each circuit
+            n = specs[i][0]
+            s = states.setdefault(n, H.HipStateVector.zero_state(n, prec))
+            s.set_basis(0)
+            s.run(circs[i])
+            s.expectations(paulis[i])
+        for s in states.values():
+            s.synchronize()
+        t0 = time.perf_counter()
+        maxdiff = 0.0
+        for i in idx:
+            s = states[specs[i][0]]
+            s.set_basis(0)
+            s.run(circs[i])
+            ev = s.expectations(paulis[i])
+            bv = b.result(pos[i])[0]
+            maxdiff = max(maxdiff, max(abs(a - c) for a, c in zip(ev, bv)))
+        per = (time.perf_counter() - t0) / len(idx)
+        log(f"per-circuit: {per * 1000:.2f} ms per circuit over {len(idx)} circuits; max |batch - single| = "
+            f"{maxdiff:.2e}")
+        return per, maxdiff
+
+    allidx = list(range(len(specs)))
+    b, wall, kms, cold = batched(allidx)
+    per, maxdiff = per_circuit(allidx[::args.stride], b, {i: i for i in allidx})
+    del b
+    small = [i for i in allidx if ns[i] <= 14]
+    bs, wall_s, kms_s, _ = batched(small)
+    per_s, maxdiff_s = per_circuit(small[::max(1, args.stride // 2)], bs, {i: k for k, i in enumerate(small)})
+    del bs
     amp_updates = float((gates * np.exp2(ns)).sum())
-    b = H.HipBatch(prec)
-    if args.tile_bits:
-        b.set_tile_bits(args.tile_bits)
-    for c, ps in zip(circs, paulis):
-        b.add(c, ps)
-    t0 = time.perf_counter()
-    b.run()  # plans every circuit and uploads the programs
-    cold = time.perf_counter() - t0
-    log(f"batch: first run {cold:.2f} s (planning {b.stats()['plan_ms']:.0f} ms)")
-    for _ in range(max(0, args.warmup - 1)):
-        b.run()
-    t0 = time.perf_counter()
-    kms = 0.0
-    for _ in range(args.steps):
-        b.run()
-        kms += b.stats()["kernel_ms"]
-    wall = (time.perf_counter() - t0) / args.steps
-    kms /= args.steps
-    st = b.stats()
-    log(f"batch: {wall * 1000:.1f} ms per run (kernel {kms:.1f} ms), {st['steps']:.0f} steps, {st['tiles']:.0f} tiles")
-    # per-circuit launches over a sample
-    os.environ["NCSV_JIT_MIN_QUBITS"] = "99"  # no per-circuit kernel compiles
-    sample = list(range(0, len(specs), args.stride))
-    states = {}
-    for i in sample:  # warm-up: plans cached on each circuit
-        n = specs[i][0]
-        s = states.setdefault((n, prec), H.HipStateVector.zero_state(n, prec))
-        s.set_basis(0)
-        s.run(circs[i])
-        s.expectations(paulis[i])
-    for s in states.values():
-        s.synchronize()
-    t0 = time.perf_counter()
-    maxdiff = 0.0
-    for i in sample:
-        n = specs[i][0]
-        s = states[(n, prec)]
-        s.set_basis(0)
-        s.run(circs[i])
-        ev = s.expectations(paulis[i])
-        bv = b.result(i)[0]
-        maxdiff = max(maxdiff, max(abs(a - c) for a, c in zip(ev, bv)))
-    per = (time.perf_counter() - t0) / len(sample)
-    del states
-    log(f"per-circuit: {per * 1000:.2f} ms per circuit over {len(sample)} circuits; max |batch - single| = {maxdiff:.2e}")
     cps = len(specs) / wall
     out = {"metric": "circuits/s", "value": cps, "unit": "circuits/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall * 1000, "higher_is_better": True, "scaling": "weak",
@@ -185,9 +198,13 @@ def run_batch(args):
            "config": {"workload": f"{len(specs)} circuits, 10-22 qubits, depth 10-120, 8 Paulis each",
                       "gates": int(gates.sum()), "tile_bits": args.tile_bits or None},
            "amp_updates_per_s": amp_updates / wall, "kernel_ms": kms, "first_run_s": cold,
-           "per_circuit": {"circuits_per_s": 1.0 / per, "sample": len(sample), "ms_per_circuit": per * 1000,
+           "per_circuit": {"circuits_per_s": 1.0 / per, "sample": len(allidx[::args.stride]),
+                           "ms_per_circuit": per * 1000,
                            "path": "single-state fused interpreter, one state per qubit count"},
-           "speedup_vs_per_circuit": cps * per, "max_abs_batch_vs_single": maxdiff}
+           "speedup_vs_per_circuit": cps * per, "max_abs_batch_vs_single": max(maxdiff, maxdiff_s),
+           "small_circuits": {"qubits": "10-14", "count": len(small), "circuits_per_s": len(small) / wall_s,
+                              "kernel_ms": kms_s, "per_circuit_circuits_per_s": 1.0 / per_s,
+                              "speedup_vs_per_circuit": len(small) / wall_s * per_s}}
     print(json.dumps(out), flush=True)
 
 
@@ -241,6 +258,23 @@ def run_traj(args):
     est128 = np.array([[e.real / b128.result(i)[1] for e in b128.result(i)[0]] for i in range(m)])
     diff = float(np.abs(est64[:m] - est128).max())
     log(f"check: {m} complex128 trajectories in {w128:.2f} s, max |est64 - est128| = {diff:.2e}")
+    # per-trajectory launches: the same circuit through the single-state run
+    # (noise ops split its fused segments), its own SplitMix64 draws
+    single = min(args.single, T)
+    s = H.HipStateVector.zero_state(n, H.C64)
+    ms_single = None
+    if single:
+        s.run(circ, rng=W.SplitMix64(1))  # warm-up: plans
+        s.synchronize()
+        t2 = time.perf_counter()
+        for k in range(single):
+            s.set_basis(0)
+            s.run(circ, rng=W.SplitMix64(100 + k))
+            s.expectations(ps)
+        s.synchronize()
+        ms_single = (time.perf_counter() - t2) * 1000 / single
+        log(f"per-trajectory launches: {ms_single:.1f} ms per trajectory over {single}")
+    del s
     gates = W.count_gates(text)
     out = {"metric": "trajectories/s", "value": T / wall, "unit": "trajectories/s", "n_gpus": 1, "steps": 1,
            "warmup": 1, "ms_per_step": wall * 1000, "higher_is_better": True, "scaling": "weak",
@@ -250,7 +284,9 @@ def run_traj(args):
            "amp_updates_per_s": gates * float(1 << n) * T / wall,
            "estimates": {"mean_Z": est64[:, :n].mean(axis=0).tolist()[:4],
                          "mean_ZZ": est64[:, n:].mean(axis=0).tolist()[:4]},
-           "fp64_check": {"trajectories": m, "max_abs_estimate_diff": diff, "seconds_c128": w128}}
+           "fp64_check": {"trajectories": m, "max_abs_estimate_diff": diff, "seconds_c128": w128},
+           "per_trajectory": {"ms_per_trajectory": ms_single, "trajectories": single,
+                              "speedup_batched": (ms_single / (wall * 1000 / T)) if ms_single else None}}
     print(json.dumps(out), flush=True)
 
 
@@ -267,6 +303,7 @@ def main():
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--stride", type=int, default=8)
     ap.add_argument("--trajectories", type=int, default=2000)
+    ap.add_argument("--single", type=int, default=3, help="trajectories timed through per-trajectory launches")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     ap.add_argument("--tile-bits", type=int, default=0)
     args = ap.parse_args()

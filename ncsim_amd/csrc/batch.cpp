@@ -4,10 +4,12 @@
 // results.  A host emulator runs the same programs on FP64 amplitudes so the
 // planner can be checked without a device.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <functional>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <future>
@@ -28,9 +30,12 @@ void simulate(const BArgs &A, ncsv_precision prec);
 struct StepH {
     uint32_t L = 0;
     std::vector<uint8_t> qt;  // tile qubits, then outer qubits ascending
-    std::vector<BOpD> ops;
+    std::vector<BOpD> ops;    // as planned
+    std::vector<BOpD> gops, gopsc;  // grouped; grouped without noise slots
+    std::vector<uint32_t> slots;    // noise slots of this step
     std::vector<cd> coef;
     std::vector<BPauliD> paulis;
+    std::vector<BMGroupD> mgroups;
     uint32_t flags = 0;
     bool modifies = false;
 };
@@ -110,6 +115,24 @@ struct Ev {
     uint32_t ph = 0, q = 0, ord = 0;
 };
 
+// Consecutive measurement events decided from one reduction.
+struct MGroup {
+    std::vector<uint64_t> masks;  // distinct Z masks (at most 4)
+    std::vector<Ev> evs;          // at most 16
+    uint64_t resets = 0;          // reset qubits
+    bool accepts(const Ev &e) const {
+        if (evs.size() >= 16) return false;
+        if (e.z & resets) return false;  // a reset qubit is not measured again within the run
+        if (std::find(masks.begin(), masks.end(), e.z) == masks.end() && masks.size() >= 4) return false;
+        return true;
+    }
+    void add(const Ev &e) {
+        if (std::find(masks.begin(), masks.end(), e.z) == masks.end()) masks.push_back(e.z);
+        evs.push_back(e);
+        if (e.reset) resets |= 1ull << e.q;
+    }
+};
+
 int pos_in(const StepH &s, uint32_t q) {
     for (uint32_t p = 0; p < s.L; p++)
         if (s.qt[p] == q) return (int)p;
@@ -127,6 +150,29 @@ void split_mask(const StepH &s, uint64_t m, uint32_t *lm, uint64_t *om) {
         else
             *om |= 1ull << q;
     }
+}
+
+// The group's descriptor for a step's tile layout (index into s->mgroups);
+// reset positions only where the step applies the resets.
+uint32_t group_desc(StepH *s, const MGroup &g, bool resets = true) {
+    BMGroupD d{};
+    d.D = (uint32_t)g.masks.size();
+    d.E = (uint32_t)g.evs.size();
+    for (uint32_t i = 0; i < d.D; i++) split_mask(*s, g.masks[i], &d.lm[i], &d.om[i]);
+    for (uint32_t i = 0; i < d.E; i++) {
+        const Ev &e = g.evs[i];
+        d.ev[i].d = (uint8_t)(std::find(g.masks.begin(), g.masks.end(), e.z) - g.masks.begin());
+        d.ev[i].sign = (uint8_t)(e.ph >> 1);
+        d.ev[i].ord = e.ord;
+        d.ev[i].rpos = 0xff;
+        if (e.reset && resets) {
+            const int p = pos_in(*s, e.q);
+            if (p < 0) fail(NCSV_INVALID, "batched planner: reset qubit outside the tile");
+            d.ev[i].rpos = (uint8_t)p;
+        }
+    }
+    s->mgroups.push_back(d);
+    return (uint32_t)s->mgroups.size() - 1;
 }
 
 void append_tile_ops(StepH *s, const SweepH &sw) {
@@ -150,11 +196,117 @@ void append_tile_ops(StepH *s, const SweepH &sw) {
         case TK_DM: d.kind = BK_DM; put(1); break;
         case TK_PERM1: d.kind = BK_PERM1; break;
         case TK_SWAPR: d.kind = BK_SWAP; break;
-        case TK_NSLOT: d.kind = BK_NSLOT; d.a = o.aux; break;
+        case TK_NSLOT:
+            d.kind = BK_NSLOT;
+            d.a = o.aux;
+            s->slots.push_back(o.aux);
+            break;
         default: fail(NCSV_INVALID, "batched form: unexpected tile op");
         }
         s->ops.push_back(d);
     }
+}
+
+// The 2x2 of an unconditional one-position op, or false.
+bool as_2x2(const StepH &s, const BOpD &o, uint32_t *pos, cd m[4]) {
+    static const cd ip[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
+    switch (o.kind) {
+    case BK_U1:
+    case BK_U1R:
+    case BK_U1X:
+        *pos = o.p0;
+        for (int t = 0; t < 4; t++) m[t] = s.coef[o.coef + t];
+        return true;
+    case BK_DM:
+        if (o.omask || __builtin_popcount(o.lmask) != 1) return false;
+        *pos = (uint32_t)__builtin_ctz(o.lmask);
+        m[0] = cd(1, 0);
+        m[1] = m[2] = cd(0, 0);
+        m[3] = s.coef[o.coef];
+        return true;
+    case BK_PERM1:
+        if (o.omask || o.lmask) return false;
+        *pos = o.p0;
+        m[0] = m[3] = cd(0, 0);
+        m[1] = ip[o.k1 & 3];
+        m[2] = ip[o.k0 & 3];
+        return true;
+    default: return false;
+    }
+}
+
+// Runs of unconditional one-position ops become one pass over up to three
+// distinct positions (a later op on a position already in the run is
+// multiplied into its 2x2; the run's positions commute).  Noise slots are
+// kept, or dropped for the variant run by entries with no error here.
+std::vector<BOpD> group_ops(StepH *s, bool drop_slots) {
+    std::vector<BOpD> out;
+    std::vector<uint32_t> pos;
+    std::vector<std::array<cd, 4>> mats;
+    std::vector<BOpD> orig;
+    auto flush = [&]() {
+        if (pos.size() >= 2) {
+            BOpD d{};
+            d.kind = BK_G3;
+            d.k1 = (uint8_t)pos.size();
+            d.p0 = (uint8_t)pos[0];
+            d.p1 = (uint8_t)pos[1];
+            d.p2 = (uint8_t)(pos.size() > 2 ? pos[2] : 0);
+            d.coef = (uint32_t)s->coef.size();
+            for (auto &m : mats)
+                for (int t = 0; t < 4; t++) s->coef.push_back(m[t]);
+            out.push_back(d);
+        } else {
+            out.insert(out.end(), orig.begin(), orig.end());
+        }
+        pos.clear();
+        mats.clear();
+        orig.clear();
+    };
+    for (const BOpD &o : s->ops) {
+        if (drop_slots && o.kind == BK_NSLOT) continue;
+        uint32_t p;
+        cd m[4];
+        if (!as_2x2(*s, o, &p, m)) {
+            flush();
+            out.push_back(o);
+            continue;
+        }
+        const auto it = std::find(pos.begin(), pos.end(), p);
+        if (it != pos.end()) {
+            // later x earlier on the same position
+            std::array<cd, 4> &e = mats[it - pos.begin()];
+            const std::array<cd, 4> r = {m[0] * e[0] + m[1] * e[2], m[0] * e[1] + m[1] * e[3],
+                                         m[2] * e[0] + m[3] * e[2], m[2] * e[1] + m[3] * e[3]};
+            e = r;
+            orig.clear();  // the run no longer equals its ops one by one
+            if (pos.size() == 1) {
+                // a single merged 2x2: keep it as a U1
+                BOpD d{};
+                d.kind = BK_U1;
+                d.p0 = (uint8_t)p;
+                d.coef = (uint32_t)s->coef.size();
+                for (int t = 0; t < 4; t++) s->coef.push_back(e[t]);
+                orig.push_back(d);
+            }
+            continue;
+        }
+        if (pos.size() == 3) flush();
+        if (pos.size() == 1 && orig.empty()) {
+            // the first position's op was a merged 2x2: restore its record
+            BOpD d{};
+            d.kind = BK_U1;
+            d.p0 = (uint8_t)pos[0];
+            d.coef = (uint32_t)s->coef.size();
+            for (int t = 0; t < 4; t++) s->coef.push_back(mats[0][t]);
+            orig.push_back(d);
+        }
+        pos.push_back(p);
+        mats.push_back({m[0], m[1], m[2], m[3]});
+        orig.push_back(o);
+    }
+    flush();
+    return out;
 }
 
 }  // namespace
@@ -291,6 +443,29 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         s.ops.push_back(d);
     };
 
+    // gate runs and measurement groups
+    struct Seg {
+        bool meas = false;
+        std::vector<LOp> ops;
+        MGroup grp;
+    };
+    std::vector<Seg> segs;
+    for (Ev &e : evs) {
+        if (!e.meas) {
+            if (e.ops.empty()) continue;
+            Seg g;
+            g.ops = std::move(e.ops);
+            segs.push_back(std::move(g));
+            continue;
+        }
+        if (segs.empty() || !segs.back().meas || !segs.back().grp.accepts(e)) {
+            Seg g;
+            g.meas = true;
+            segs.push_back(std::move(g));
+        }
+        segs.back().grp.add(e);
+    }
+
     if (n <= Lb) {
         // one step on one tile: the state stays in LDS
         StepH s;
@@ -298,10 +473,9 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         for (uint32_t q = 0; q < n; q++) s.qt.push_back((uint8_t)q);
         s.flags = BF_INIT;
         s.modifies = true;
-        for (const Ev &e : evs) {
-            if (!e.meas) {
-                if (e.ops.empty()) continue;
-                for (const SweepH &sw : plan_lds_sweeps(e.ops, n, n, cl, 0)) {
+        for (const Seg &g : segs) {
+            if (!g.meas) {
+                for (const SweepH &sw : plan_lds_sweeps(g.ops, n, n, cl, 0)) {
                     for (uint32_t p = 0; p < n; p++)
                         if (sw.tq[p] != p) fail(NCSV_INVALID, "batched planner: whole-state tile not in order");
                     append_tile_ops(&s, sw);
@@ -309,19 +483,17 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             } else {
                 BOpD d{};
                 d.kind = BK_MEASURE;
-                d.lmask = (uint32_t)e.z;
-                d.k0 = (uint8_t)(e.ph >> 1);
-                d.c = e.ord;
-                d.p0 = e.reset ? (uint8_t)e.q : (uint8_t)0xff;
+                d.c = group_desc(&s, g.grp);
                 s.ops.push_back(d);
             }
         }
         P->steps.push_back(std::move(s));
         for (const Pg &g : groups) add_expect(P->steps.back(), g, 1);
     } else {
+        // a group reduced at the end of one step is collapsed at the start of the next
         struct Pend {
             bool on = false;
-            Ev ev;
+            const MGroup *grp = nullptr;
             uint32_t off = 0, tiles = 0;
         } pend;
         const uint32_t tiles = 1u << (n - L);
@@ -333,6 +505,7 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             for (uint64_t a = T; a; a &= a - 1) tq.push_back((uint32_t)__builtin_ctzll(a));
             return tq;
         };
+        auto pend_resets = [&]() { return pend.on ? pend.grp->resets : 0ull; };
         auto make_step = [&](const std::vector<uint32_t> &tq) -> StepH & {
             StepH s;
             s.L = L;
@@ -351,15 +524,7 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
                 d.kind = BK_COLLAPSE;
                 d.a = pend.off;
                 d.b = pend.tiles;
-                d.c = pend.ev.ord;
-                split_mask(s, pend.ev.z, &d.lmask, &d.omask);
-                d.k0 = (uint8_t)(pend.ev.ph >> 1);
-                d.p0 = 0xff;
-                if (pend.ev.reset) {
-                    const int p = pos_in(s, pend.ev.q);
-                    if (p < 0) fail(NCSV_INVALID, "batched planner: reset qubit outside the tile");
-                    d.p0 = (uint8_t)p;
-                }
+                d.c = group_desc(&s, *pend.grp);
                 s.ops.push_back(d);
                 s.modifies = true;
                 pend.on = false;
@@ -367,30 +532,28 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             P->steps.push_back(std::move(s));
             return P->steps.back();
         };
-        for (const Ev &e : evs) {
-            if (!e.meas) {
-                if (e.ops.empty()) continue;
-                const uint64_t ff = (pend.on && pend.ev.reset) ? 1ull << pend.ev.q : 0;
-                for (const SweepH &sw : plan_lds_sweeps(e.ops, n, L, cl, ff)) {
+        for (const Seg &g : segs) {
+            if (!g.meas) {
+                for (const SweepH &sw : plan_lds_sweeps(g.ops, n, L, cl, pend_resets())) {
                     StepH &s = make_step(sw.tq);
                     append_tile_ops(&s, sw);
                     s.modifies = true;
                 }
             } else {
-                if (P->steps.empty() || pend.on)
-                    make_step(default_tq(pend.on && pend.ev.reset ? 1ull << pend.ev.q : 0));
+                if ((uint32_t)__builtin_popcountll(g.grp.resets) > L)
+                    fail(NCSV_UNSUPPORTED, "batched form: more reset qubits in one run than tile qubits");
+                if (P->steps.empty() || pend.on) make_step(default_tq(pend_resets()));
                 StepH &s = P->steps.back();
                 BOpD d{};
                 d.kind = BK_REDUCE;
                 d.a = (uint32_t)P->npart;
-                split_mask(s, e.z, &d.lmask, &d.omask);
-                d.k0 = (uint8_t)(e.ph >> 1);
+                d.c = group_desc(&s, g.grp, false);
                 s.ops.push_back(d);
                 pend.on = true;
-                pend.ev = e;
+                pend.grp = &g.grp;
                 pend.off = (uint32_t)P->npart;
                 pend.tiles = tiles;
-                P->npart += 2ull * tiles;
+                P->npart += (uint64_t)(1u << g.grp.masks.size()) * tiles;
             }
         }
         auto fits_last = [&](uint64_t xs) {
@@ -402,10 +565,10 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         for (const Pg &g : groups) {
             if (!fits_last(g.xs)) {
                 uint64_t need = g.xs;
-                if (pend.on && pend.ev.reset) {
-                    const uint64_t rq = 1ull << pend.ev.q;
+                if (pend.on && pend.grp->resets) {
+                    const uint64_t rq = pend.grp->resets;
                     if ((uint32_t)__builtin_popcountll(need | rq) > L)
-                        make_step(default_tq(rq));  // the reset's collapse on its own
+                        make_step(default_tq(rq));  // the resets' collapse on its own
                     else
                         need |= rq;
                 }
@@ -415,6 +578,24 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         }
         for (size_t i = 0; i + 1 < P->steps.size(); i++)
             if (P->steps[i].modifies) P->steps[i].flags |= BF_STORE;
+    }
+    for (StepH &st : P->steps) {
+        st.gops = group_ops(&st, false);
+        st.gopsc = group_ops(&st, true);
+    }
+    if (getenv("NCSV_BATCH_DUMP")) {
+        static const char *nm[] = {"U1", "U1R", "U1X", "U2", "DM", "PERM1", "SWAP", "NSLOT",
+                                   "MEAS", "RED", "COLL", "EXP", "G3"};
+        for (size_t i = 0; i < P->steps.size(); i++) {
+            const StepH &st = P->steps[i];
+            int cnt[13] = {0};
+            for (const BOpD &o : st.gopsc) cnt[o.kind]++;
+            fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu:", i, st.L, st.flags, st.gops.size(),
+                    st.gopsc.size(), st.slots.size());
+            for (int k = 0; k < 13; k++)
+                if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
+            fprintf(stderr, "\n");
+        }
     }
     for (uint32_t s = 0; s < P->steps.size(); s++) {
         P->items += P->tiles(s);
@@ -450,7 +631,7 @@ struct ncsv_batch {
     // device program tables
     void *d_tab = nullptr;
     size_t tab_bytes = 0;
-    size_t off_steps = 0, off_ops = 0, off_coef = 0, off_paulis = 0, off_qtab = 0;
+    size_t off_steps = 0, off_ops = 0, off_coef = 0, off_paulis = 0, off_mgroups = 0, off_qtab = 0;
     bool tables_dirty = true;
     // per-chunk device buffers (grow only)
     void *d_amp = nullptr;
@@ -554,6 +735,7 @@ struct Tables {
     std::vector<BOpD> ops;
     std::vector<double> coef;
     std::vector<BPauliD> paulis;
+    std::vector<BMGroupD> mgroups;
     std::vector<uint8_t> qtab;
 };
 
@@ -565,21 +747,30 @@ Tables flatten(ncsv_batch *b) {
             BStepD d{};
             d.L = s.L;
             d.n = P->n;
-            d.op0 = (uint32_t)t.ops.size();
-            d.nops = (uint32_t)s.ops.size();
             d.flags = s.flags;
             d.qoff = (uint32_t)t.qtab.size();
-            const uint32_t c0 = (uint32_t)(t.coef.size() / 2), p0 = (uint32_t)t.paulis.size();
-            for (BOpD o : s.ops) {
-                if (o.kind <= BK_DM) o.coef += c0;
-                if (o.kind == BK_EXPECT) o.c += p0;
-                t.ops.push_back(o);
-            }
+            const uint32_t c0 = (uint32_t)(t.coef.size() / 2), p0 = (uint32_t)t.paulis.size(),
+                           g0 = (uint32_t)t.mgroups.size();
+            auto put = [&](const std::vector<BOpD> &ops) {
+                for (BOpD o : ops) {
+                    if (o.kind <= BK_DM || o.kind == BK_G3) o.coef += c0;
+                    if (o.kind == BK_EXPECT) o.c += p0;
+                    if (o.kind == BK_MEASURE || o.kind == BK_REDUCE || o.kind == BK_COLLAPSE) o.c += g0;
+                    t.ops.push_back(o);
+                }
+            };
+            d.op0 = (uint32_t)t.ops.size();
+            d.nops = (uint32_t)s.gops.size();
+            put(s.gops);
+            d.op0c = (uint32_t)t.ops.size();
+            d.nopsc = (uint32_t)s.gopsc.size();
+            put(s.gopsc);
             for (const cd &c : s.coef) {
                 t.coef.push_back(c.real());
                 t.coef.push_back(c.imag());
             }
             t.paulis.insert(t.paulis.end(), s.paulis.begin(), s.paulis.end());
+            t.mgroups.insert(t.mgroups.end(), s.mgroups.begin(), s.mgroups.end());
             t.qtab.insert(t.qtab.end(), s.qt.begin(), s.qt.end());
             t.steps.push_back(d);
         }
@@ -597,6 +788,8 @@ void upload_tables(ncsv_batch *b, const Tables &t) {
     off = up256(off + t.coef.size() * sizeof(double));
     b->off_paulis = off;
     off = up256(off + t.paulis.size() * sizeof(BPauliD));
+    b->off_mgroups = off;
+    off = up256(off + t.mgroups.size() * sizeof(BMGroupD));
     b->off_qtab = off;
     off = up256(off + t.qtab.size() + 1);
     if (off > b->tab_bytes) {
@@ -610,6 +803,7 @@ void upload_tables(ncsv_batch *b, const Tables &t) {
     memcpy(h.data() + b->off_ops, t.ops.data(), t.ops.size() * sizeof(BOpD));
     memcpy(h.data() + b->off_coef, t.coef.data(), t.coef.size() * sizeof(double));
     memcpy(h.data() + b->off_paulis, t.paulis.data(), t.paulis.size() * sizeof(BPauliD));
+    memcpy(h.data() + b->off_mgroups, t.mgroups.data(), t.mgroups.size() * sizeof(BMGroupD));
     memcpy(h.data() + b->off_qtab, t.qtab.data(), t.qtab.size());
     check_hip(hipMemcpy(b->d_tab, h.data(), off, hipMemcpyHostToDevice), "upload batch programs");
     b->tables_dirty = false;
@@ -663,6 +857,13 @@ void fill_draws(const Program &P, const double *u, uint64_t *seed, const uint8_t
 
 Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *uniforms,
                  const std::vector<size_t> &draw_off, const uint64_t *seeds, const uint8_t *forced) {
+    // tiles per queue item: fewer, longer items mean fewer device-scope releases
+    uint32_t lg_tiles_per_item = 3;
+    if (const char *e = getenv("NCSV_BATCH_TILES_PER_ITEM")) {
+        const int v = atoi(e);
+        lg_tiles_per_item = 0;
+        while (v > 1 && (1 << (lg_tiles_per_item + 1)) <= v) lg_tiles_per_item++;
+    }
     Chunk ch;
     ch.ent = ent;
     const size_t ne = ent.size();
@@ -709,9 +910,15 @@ Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *
             r.step = P.step0 + d;
             r.prev = last[i];
             r.prev_tiles = d ? P.tiles(d - 1) : 0;
+            uint32_t err = 0;
+            for (uint32_t sl : P.steps[d].slots) err |= ch.errs[ch.ed[i].err0 + sl];
+            r.flags = (err ? RF_ERR : 0u) | (d + 1 < P.steps.size() ? RF_RELEASE : 0u);
+            // at least 8 items per step where the step has that many tiles
+            const uint32_t lg = P.n - P.steps[d].L;
+            r.lgt = std::min(lg > 3 ? lg - 3 : 0u, lg_tiles_per_item);
             last[i] = (uint32_t)ch.recs.size();
             ch.recs.push_back(r);
-            ch.total += P.tiles(d);
+            ch.total += P.tiles(d) >> r.lgt;
         }
     }
     BRecD sentinel{};
@@ -766,6 +973,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
     A.ops = at<const BOpD>(b->d_tab, b->off_ops);
     A.coef = at<const double>(b->d_tab, b->off_coef);
     A.paulis = at<const BPauliD>(b->d_tab, b->off_paulis);
+    A.mgroups = at<const BMGroupD>(b->d_tab, b->off_mgroups);
     A.qtab = at<const uint8_t>(b->d_tab, b->off_qtab);
     A.entries = (const BEntryD *)(w + o_ent);
     A.recs = (const BRecD *)(w + o_rec);
@@ -819,6 +1027,7 @@ void run_chunk_sim(ncsv_batch *b, const Tables &t, Chunk &ch) {
     A.ops = t.ops.data();
     A.coef = t.coef.data();
     A.paulis = t.paulis.data();
+    A.mgroups = t.mgroups.data();
     A.qtab = t.qtab.data();
     A.entries = ch.ed.data();
     A.recs = ch.recs.data();
@@ -888,10 +1097,25 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                 cdv tl(NL);
                 for (uint32_t x = 0; x < NL; x++)
                     tl[x] = (st.flags & BF_INIT) ? cd((x == 0 && tileno == 0) ? 1.0 : 0.0, 0.0) : amp[addr[x]];
-                for (uint32_t oi = st.op0; oi < st.op0 + st.nops; oi++) {
+                uint32_t err = 0;
+                for (uint32_t sl : P.steps[si].slots) err |= ch.errs[en.err0 + sl];
+                const uint32_t op0 = err ? st.op0 : st.op0c, op1 = op0 + (err ? st.nops : st.nopsc);
+                for (uint32_t oi = op0; oi < op1; oi++) {
                     const BOpD &op = t.ops[oi];
                     auto C = [&](uint32_t k) { return cd(t.coef[2 * (op.coef + k)], t.coef[2 * (op.coef + k) + 1]); };
                     switch (op.kind) {
+                    case BK_G3: {
+                        const uint32_t ps[3] = {op.p0, op.p1, op.p2};
+                        for (uint32_t i = 0; i < op.k1; i++)
+                            for (uint32_t x0 = 0; x0 < NL; x0++) {
+                                if ((x0 >> ps[i]) & 1) continue;
+                                const uint32_t x1 = x0 | (1u << ps[i]);
+                                const cd a = tl[x0], v = tl[x1];
+                                tl[x0] = C(4 * i) * a + C(4 * i + 1) * v;
+                                tl[x1] = C(4 * i + 2) * a + C(4 * i + 3) * v;
+                            }
+                        break;
+                    }
                     case BK_U1:
                     case BK_U1R:
                     case BK_U1X:
@@ -952,54 +1176,65 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                     case BK_MEASURE:
                     case BK_REDUCE:
                     case BK_COLLAPSE: {
-                        const uint32_t sgn = op.k0 ^ parity(base & op.omask);
-                        double nn = 0, e = 0;
+                        const BMGroupD &G = t.mgroups[op.c];
+                        const uint32_t NC = 1u << G.D;
+                        uint32_t ocell = 0;
+                        for (uint32_t d = 0; d < G.D; d++) ocell |= parity(base & G.om[d]) << d;
+                        auto cell = [&](uint32_t x) {
+                            uint32_t c = ocell;
+                            for (uint32_t d = 0; d < G.D; d++) c ^= parity(x & G.lm[d]) << d;
+                            return c;
+                        };
+                        double W[16] = {0};
                         if (op.kind == BK_COLLAPSE) {
-                            for (uint32_t tt = 0; tt < op.b; tt++) {
-                                nn += part[en.part0 + op.a + 2 * tt];
-                                e += part[en.part0 + op.a + 2 * tt + 1];
-                            }
+                            for (uint32_t tt = 0; tt < op.b; tt++)
+                                for (uint32_t c = 0; c < NC; c++) W[c] += part[en.part0 + op.a + NC * tt + c];
                         } else {
-                            for (uint32_t x = 0; x < NL; x++) {
-                                const double w = std::norm(tl[x]);
-                                nn += w;
-                                e += (parity(x & op.lmask) ^ sgn) ? -w : w;
-                            }
+                            for (uint32_t x = 0; x < NL; x++) W[cell(x)] += std::norm(tl[x]);
                         }
                         if (op.kind == BK_REDUCE) {
-                            part[en.part0 + op.a + 2 * tileno] = nn;
-                            part[en.part0 + op.a + 2 * tileno + 1] = e;
+                            for (uint32_t c = 0; c < NC; c++) part[en.part0 + op.a + NC * tileno + c] = W[c];
                             break;
                         }
-                        const uint32_t ord = en.m0 + op.c;
-                        double p_plus = (1.0 + e / nn) / 2.0;
-                        if (!(p_plus >= 0.0)) p_plus = 0.0;
-                        if (p_plus > 1.0) p_plus = 1.0;
-                        uint32_t o;
-                        if (ch.forced[ord] != 0xff)
-                            o = ch.forced[ord] & 1;
-                        else if (p_plus > 1.0 - 1e-12)
-                            o = 0;
-                        else if (p_plus < 1e-12)
-                            o = 1;
-                        else
-                            o = ch.uni[ord] >= p_plus ? 1 : 0;
-                        const double kept = 0.5 * (nn + (o ? -e : e));
-                        double sc = 1.0;
-                        if (c64 && kept > 0.0 && kept < 0.25) sc = 1.0 / std::sqrt(kept);
-                        if (tileno == 0) {
-                            ch.outc[ord] = (uint8_t)o;
-                            if (sc != 1.0) ch.lsc[i] *= std::sqrt(kept);
-                        }
-                        if (op.p0 == 0xff) {
-                            for (uint32_t x = 0; x < NL; x++)
-                                tl[x] = ((parity(x & op.lmask) ^ sgn) != o) ? cd(0, 0) : tl[x] * sc;
-                        } else {
-                            for (uint32_t j = 0; j < NL / 2; j++) {
-                                const uint32_t x0 = ins0h(j, op.p0), x1 = x0 | (1u << op.p0);
-                                tl[x0] = (o ? tl[x1] : tl[x0]) * sc;
-                                tl[x1] = 0;
+                        uint32_t allow = (1u << NC) - 1, flip = 0;
+                        double scale = 1.0, lfac = 1.0;
+                        for (uint32_t e = 0; e < G.E; e++) {
+                            const auto &ev = G.ev[e];
+                            double nn = 0, ee = 0;
+                            for (uint32_t c = 0; c < NC; c++) {
+                                if (!((allow >> c) & 1)) continue;
+                                nn += W[c];
+                                ee += (((c >> ev.d) & 1) ^ ev.sign) ? -W[c] : W[c];
                             }
+                            const uint32_t ord = en.m0 + ev.ord;
+                            double p_plus = (1.0 + ee / nn) / 2.0;
+                            if (!(p_plus >= 0.0)) p_plus = 0.0;
+                            if (p_plus > 1.0) p_plus = 1.0;
+                            uint32_t o;
+                            if (ch.forced[ord] != 0xff)
+                                o = ch.forced[ord] & 1;
+                            else if (p_plus > 1.0 - 1e-12)
+                                o = 0;
+                            else if (p_plus < 1e-12)
+                                o = 1;
+                            else
+                                o = ch.uni[ord] >= p_plus ? 1 : 0;
+                            for (uint32_t c = 0; c < NC; c++)
+                                if ((((c >> ev.d) & 1) ^ ev.sign) != o) allow &= ~(1u << c);
+                            const double kept = 0.5 * (nn + (o ? -ee : ee)) * scale * scale;
+                            if (c64 && kept > 0.0 && kept < 0.25) {
+                                scale /= std::sqrt(kept);
+                                lfac *= std::sqrt(kept);
+                            }
+                            if (tileno == 0) ch.outc[ord] = (uint8_t)o;
+                            if (ev.rpos != 0xff && o) flip ^= 1u << ev.rpos;
+                        }
+                        if (tileno == 0) ch.lsc[i] *= lfac;
+                        for (uint32_t x = 0; x < NL; x++) tl[x] = ((allow >> cell(x)) & 1) ? tl[x] * scale : cd(0, 0);
+                        if (flip) {
+                            cdv nt(NL);
+                            for (uint32_t x = 0; x < NL; x++) nt[x] = tl[x ^ flip];
+                            tl.swap(nt);
                         }
                         break;
                     }

@@ -16,7 +16,7 @@ template <> struct V2<double> { using type = double2; };
 }  // namespace ncsv
 
 #define NCSV_DEV __device__
-#define NCSV_KERNEL __global__ __launch_bounds__(BT)
+#define NCSV_KERNEL __global__ __launch_bounds__(BT, 4)
 #define NCSV_SHARED __shared__
 #define NCSV_TID threadIdx.x
 #define NCSV_SYNC() __syncthreads()
@@ -31,6 +31,7 @@ template <> struct V2<double> { using type = double2; };
 #define NCSV_UNIFORM64(v) ::ncsv::batch::kern::uniform64(v)
 #define NCSV_UNIFORM32(v) __builtin_amdgcn_readfirstlane((uint32_t)(v))
 #define NCSV_LANE0_STORE(dst, v) ((dst) = ::ncsv::batch::kern::uniform64(v))
+#define NCSV_OPAQUE(x) asm volatile("" : "+v"(x))
 
 namespace ncsv {
 namespace batch {
@@ -80,9 +81,11 @@ void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipS
     hipLaunchKernelGGL(k_batch_reset, dim3((nrec + 255) / 256), dim3(256), 0, stream, A.done, nrec, A.next, A.flag);
     check_hip(hipGetLastError(), "k_batch_reset");
     if (prec == NCSV_C128)
-        hipLaunchKernelGGL((k_batch<double, 12>), dim3(blocks), dim3(BT), 0, stream, A);
+        hipLaunchKernelGGL((k_batch<double, 12>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
+                           A.qtab, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
     else
-        hipLaunchKernelGGL((k_batch<float, 13>), dim3(blocks), dim3(BT), 0, stream, A);
+        hipLaunchKernelGGL((k_batch<float, 13>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
+                           A.qtab, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
     check_hip(hipGetLastError(), "k_batch");
 }
 

@@ -16,10 +16,12 @@
 // An entry of n <= L qubits is one step on one tile: its state never leaves
 // LDS.  Larger entries keep their state in HBM between steps.
 //
-// The launch claims (step, tile) items from one counter in depth-major
-// order (step d of every entry before step d+1 of any), and an item waits
-// for all tiles of its entry's previous step through that step's completion
-// counter; the lowest unfinished item never waits, so the queue drains.
+// The launch claims items (a step's run of 2^lgt tiles) from one counter in
+// depth-major order (step d of every entry before step d+1 of any), and an
+// item waits for all tiles of its entry's previous step through that step's
+// completion counter; the lowest unfinished item never waits, so the queue
+// drains.  Only steps that a later step waits on publish their stores with
+// a device-scope release (once per item).
 #pragma once
 
 #include <cstdint>
@@ -36,14 +38,15 @@ enum BOpKind : uint8_t {
     BK_PERM1 = 5,   // flip p0 where the conditions hold: new1 = i^k0 v0, new0 = i^k1 v1
     BK_SWAP = 6,    // swap positions p0 < p1
     BK_NSLOT = 7,   // noise slot a on position p0: error code bit0 = X, bit1 = Z (Z first)
-    BK_MEASURE = 8, // whole tile: measure Z-string (lmask, sign k0), ordinal a, reset p0 (0xff none)
-    BK_REDUCE = 9,  // partials (norm, <Z-string>) to part[a + 2 tile]; lmask/omask = z, k0 = sign
-    BK_COLLAPSE = 10,  // from b tiles of partials at a; ordinal c; lmask/omask = z, k0 = sign; reset p0
+    BK_MEASURE = 8, // whole tile: measurement group c (reduce, decide, collapse, resets)
+    BK_REDUCE = 9,  // group c's cell weights to part[a + 2^D tile]
+    BK_COLLAPSE = 10,  // group c decided from b tiles of cell weights at a; collapse, resets
     BK_EXPECT = 11, // Paulis [c, c+b) to part[a + 2b tile + 2k]
+    BK_G3 = 12,     // k1 (2 or 3) unconditional 2x2s on distinct positions p0, p1, p2 in one pass
 };
 
 struct BOpD {
-    uint8_t kind, p0, p1, k0, k1, pad0, pad1, pad2;
+    uint8_t kind, p0, p1, k0, k1, p2, pad1, pad2;
     uint32_t lmask;  // condition / Z mask on tile positions
     uint32_t a, b, c;
     uint32_t coef;   // first complex coefficient
@@ -56,14 +59,29 @@ struct BPauliD {
     uint32_t ph, pad;
 };
 
+// A run of consecutive Z-type measurements and resets decided together: the
+// weights of the 2^D cells (parities of the D distinct Z masks) are reduced
+// once; the events then take their outcomes in order from those weights,
+// each restricting the kept cells, and resets flip their qubit at the end
+// (no later event of the run involves a reset qubit).
+struct BMGroupD {
+    uint32_t D, E;
+    uint32_t lm[4];  // masks on tile positions
+    uint64_t om[4];  // masks on outer qubits
+    struct Ev {
+        uint8_t d, sign, rpos, pad;  // mask index, sign (ph / 2), reset position or 0xff
+        uint32_t ord;                // measurement ordinal
+    } ev[16];
+};
+
 enum BStepFlags : uint32_t { BF_INIT = 1, BF_LOAD = 2, BF_STORE = 4 };
 
 struct BStepD {
     uint32_t L, n;
-    uint32_t op0, nops;
+    uint32_t op0, nops;    // the ops
+    uint32_t op0c, nopsc;  // the same without noise slots (entries with no error drawn in this step)
     uint32_t flags;
     uint32_t qoff;  // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
-    uint32_t pad[2];
 };
 
 struct BEntryD {
@@ -79,13 +97,18 @@ struct BRecD {
     uint32_t entry, step;  // step: index into the step table
     uint32_t prev;        // record of the entry's previous step, or ~0u
     uint32_t prev_tiles;  // its tile count
+    uint32_t flags;       // RF_ERR: a noise error was drawn in this step; RF_RELEASE: a later step waits on it
+    uint32_t lgt;         // log2 of the tiles per item
 };
+
+enum BRecFlags : uint32_t { RF_ERR = 1, RF_RELEASE = 2 };
 
 struct BArgs {
     const BStepD *steps;
     const BOpD *ops;
     const double *coef;  // complex coefficients as (re, im)
     const BPauliD *paulis;
+    const BMGroupD *mgroups;
     const uint8_t *qtab;
     const BEntryD *entries;
     const BRecD *recs;        // + sentinel

@@ -8,7 +8,15 @@
 //   NCSV_LOAD_AGENT(p), NCSV_ADD_AGENT(p, v), NCSV_FENCE_ACQUIRE(),
 //   NCSV_FENCE_RELEASE(), NCSV_SLEEP(), NCSV_UNIFORM32(v) / NCSV_UNIFORM64(v)
 //   (the value of the wavefront's first lane), NCSV_LANE0_STORE(dst, v)
-//   (dst = lane 0's v), V2<T>::type (float2 / double2).
+//   (dst = lane 0's v), NCSV_OPAQUE(x) (x unknown to the optimizer from
+//   here on), V2<T>::type (float2 / double2).
+//
+// The tile sits in LDS at swizzled slots: slot(x) = x ^ f(x >> S), an
+// XOR-linear bijection that spreads the strided patterns of pair, quad and
+// group accesses over the banks.  Every index used by an op is a per-thread
+// part XOR a wave-uniform part (both swizzled), so an access costs one XOR.
+// The read-only tables arrive as separate restrict-qualified arguments so
+// their wave-uniform reads go through the scalar cache.
 #pragma once
 
 #include "batch.hpp"
@@ -17,12 +25,10 @@ namespace ncsv {
 namespace batch {
 namespace kern {
 
-
 constexpr uint32_t BT = 512;        // threads per workgroup
 constexpr uint32_t NW = BT / 64;    // wavefronts per workgroup
 constexpr uint32_t LOB = 9;         // log2(BT): tile bits taken from the thread index
 constexpr uint32_t CMAX = 18;       // values per block reduction (9 Paulis)
-
 
 template <typename V> NCSV_DEV inline V cm(V a, V b) {
     V r;
@@ -64,8 +70,37 @@ NCSV_DEV inline uint32_t deposit(uint32_t j, uint32_t mask, uint32_t ones) {
     }
     return j;
 }
+template <uint32_t N> struct KTag {
+    static constexpr uint32_t value = N;
+};
+
 NCSV_DEV inline uint32_t par32(uint32_t v) { return (uint32_t)__builtin_popcount(v) & 1u; }
 NCSV_DEV inline uint32_t par64(uint64_t v) { return (uint32_t)__builtin_popcountll(v) & 1u; }
+
+// LDS slot of tile index x (rows of 256 bytes: 32 float2 or 16 double2)
+template <typename V> NCSV_DEV inline uint32_t sw(uint32_t x) {
+    constexpr uint32_t S = sizeof(V) == 16 ? 4u : 5u, M = (1u << S) - 1;
+    return x ^ (((x >> S) ^ (x >> (2 * S))) & M);
+}
+
+// f(x, slot) for every x = deposit(j, mask, ones), j < NL >> popcount(mask).
+// deposit and the swizzle are linear over disjoint bits, so x splits into
+// the thread's part (j's low 9 bits) and a wave-uniform part per iteration.
+template <typename V, typename F>
+NCSV_DEV inline void visit(uint32_t NL, uint32_t tid, uint32_t mask, uint32_t ones, F &&f) {
+    const uint32_t cnt = NL >> __builtin_popcount(mask);
+    const uint32_t xt = deposit(tid, mask, ones);
+    if (cnt <= BT) {
+        if (tid < cnt) f(xt, sw<V>(xt));
+        return;
+    }
+    const uint32_t st = sw<V>(xt);
+#pragma unroll 2
+    for (uint32_t i = 0; i < cnt / BT; i++) {
+        const uint32_t h = deposit(i * BT, mask, 0);
+        f(xt | h, st ^ sw<V>(h));
+    }
+}
 
 // Sums v[0..cnt) over the workgroup (fixed order) into bc[0..cnt), which
 // stays valid until the next barrier-separated reduction.
@@ -94,19 +129,25 @@ NCSV_DEV inline void block_sum(double (&v)[CMAX], uint32_t cnt, double *red, dou
 }
 
 // QuantumState::measure's decision (mod.rs:113-131) from FP64 sums.
-NCSV_DEV inline uint32_t decide(const BArgs &A, uint32_t ordinal, double nn, double e) {
+NCSV_DEV inline uint32_t decide(const double *__restrict__ uni, const uint8_t *__restrict__ forced, uint32_t ordinal,
+                                double nn, double e) {
     double p_plus = (1.0 + e / nn) / 2.0;
     if (!(p_plus >= 0.0)) p_plus = 0.0;
     if (p_plus > 1.0) p_plus = 1.0;
-    const uint8_t f = A.forced ? A.forced[ordinal] : (uint8_t)0xff;
+    const uint8_t f = forced ? forced[ordinal] : (uint8_t)0xff;
     if (f != 0xff) return f & 1u;
     if (p_plus > 1.0 - 1e-12) return 0;
     if (p_plus < 1e-12) return 1;
-    return A.uni[ordinal] >= p_plus ? 1u : 0u;
+    return uni[ordinal] >= p_plus ? 1u : 0u;
 }
 
 template <typename T, uint32_t LB>
-NCSV_KERNEL void k_batch(BArgs A) {
+NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restrict__ ops,
+                         const double *__restrict__ cf, const BPauliD *__restrict__ paulis,
+                         const BMGroupD *__restrict__ mgroups, const uint8_t *__restrict__ qtab, const BEntryD *__restrict__ entries,
+                         const BRecD *__restrict__ recs, const uint32_t *__restrict__ rec_blk,
+                         const double *__restrict__ uni, const uint8_t *__restrict__ forced,
+                         const uint8_t *__restrict__ errs, BArgs A) {
     using V = typename V2<T>::type;
     NCSV_SHARED V tile[1u << LB];
     NCSV_SHARED double red[NW * CMAX];
@@ -119,6 +160,9 @@ NCSV_KERNEL void k_batch(BArgs A) {
     // barriers lets the compiler split one lane from the rest of its wave
     // across iterations, which deadlocks the workgroup.
     const uint32_t wave = NCSV_UNIFORM32(tid >> 6);
+    constexpr uint32_t KH = (1u << LB) / BT;  // amplitudes per thread in a full tile
+    constexpr uint32_t HB = LB - LOB;
+    const uint32_t st_tid = sw<V>(tid);
     for (;;) {
         if (wave == 0) {
             const unsigned long long old = NCSV_ATOMIC_ADD(A.next, tid == 0 ? 1ull : 0ull);
@@ -127,10 +171,11 @@ NCSV_KERNEL void k_batch(BArgs A) {
         NCSV_SYNC();
         const uint64_t item = NCSV_UNIFORM64(s_claim);
         if (item >= A.total) break;
-        uint32_t r = A.rec_blk[item >> 4];
-        while (A.recs[r + 1].first <= item) r++;
-        const BRecD rec = A.recs[r];
-        const uint64_t tileno = item - rec.first;
+        uint32_t r = rec_blk[item >> 4];
+        while (recs[r + 1].first <= item) r++;
+        const BRecD rec = recs[r];
+        const uint32_t ntl = 1u << rec.lgt;
+        const uint64_t t0 = (item - rec.first) << rec.lgt;
         if (rec.prev != ~0u) {
             if (wave == 0) {
                 uint32_t spins = 0;
@@ -148,291 +193,380 @@ NCSV_KERNEL void k_batch(BArgs A) {
             NCSV_FENCE_ACQUIRE();
         }
 
-        const BStepD st = A.steps[rec.step];
-        const BEntryD en = A.entries[rec.entry];
+        const BStepD st = steps[rec.step];
+        const BEntryD en = entries[rec.entry];
         const uint32_t L = st.L, NL = 1u << L;
-        const uint8_t *qt = A.qtab + st.qoff;
+        const uint8_t *qt = qtab + st.qoff;
         V *g = (V *)A.amp + en.amp_off;
-        uint64_t base = 0;
-        for (uint32_t b = 0; b + L < st.n; b++)
-            if ((tileno >> b) & 1) base |= 1ull << qt[L + b];
         uint64_t offlo = 0;
         for (uint32_t p = 0; p < L && p < LOB; p++)
             if ((tid >> p) & 1) offlo |= 1ull << qt[p];
+        // memory strides of the tile positions above the thread bits
+        uint64_t wq[HB];
+#pragma unroll
+        for (uint32_t b = 0; b < HB; b++) wq[b] = (b + LOB < L) ? 1ull << qt[LOB + b] : 0ull;
+        const uint32_t kcnt = L >= LOB ? 1u << (L - LOB) : (tid < NL ? 1u : 0u);
+        const uint32_t op0 = (rec.flags & RF_ERR) ? st.op0 : st.op0c;
+        const uint32_t op1 = op0 + ((rec.flags & RF_ERR) ? st.nops : st.nopsc);
+        // each thread reads back only the LDS slots it wrote, so consecutive
+        // tiles need no barrier between the store of one and the load of the next
+        for (uint32_t ti = 0; ti < ntl; ti++) {
+            const uint64_t tileno = t0 + ti;
+            uint64_t base = 0;
+            for (uint32_t b = 0; b + L < st.n; b++)
+                if ((tileno >> b) & 1) base |= 1ull << qt[L + b];
 
-        if (st.flags & BF_INIT) {
-            for (uint32_t x = tid; x < NL; x += BT) {
-                V v;
-                v.x = (x == 0 && tileno == 0) ? (T)1 : (T)0;
-                v.y = (T)0;
-                tile[x] = v;
+            if (st.flags & BF_INIT) {
+#pragma unroll
+                for (uint32_t k = 0; k < KH; k++) {
+                    if (k < kcnt) {
+                        V v;
+                        v.x = (k == 0 && tid == 0 && tileno == 0) ? (T)1 : (T)0;
+                        v.y = (T)0;
+                        tile[st_tid ^ sw<V>(BT * k)] = v;
+                    }
+                }
+            } else {
+                // loads issued in chunks before any of them is waited on
+                constexpr uint32_t CH = sizeof(T) == 4 ? 8u : 4u;
+                const uint64_t tb = base + offlo;
+#pragma unroll
+                for (uint32_t k0 = 0; k0 < KH; k0 += CH) {
+                    V v[CH];
+#pragma unroll
+                    for (uint32_t c = 0; c < CH; c++) {
+                        const uint32_t k = k0 + c;
+                        if (k < kcnt) {
+                            uint64_t off = tb;
+#pragma unroll
+                            for (uint32_t b = 0; b < HB; b++)
+                                if ((k >> b) & 1) off += wq[b];
+                            v[c] = g[off];
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t c = 0; c < CH; c++)
+                        if (k0 + c < kcnt) tile[st_tid ^ sw<V>(BT * (k0 + c))] = v[c];
+                }
             }
-        } else {
-            uint32_t k = 0;
-            for (uint32_t x = tid; x < NL; x += BT, k++) {
-                uint64_t off = offlo;
-                for (uint32_t b = 0; b + LOB < L; b++)
-                    if ((k >> b) & 1) off |= 1ull << qt[LOB + b];
-                tile[x] = g[base + off];
-            }
-        }
-        NCSV_SYNC();
+            NCSV_SYNC();
 
-        for (uint32_t oi = st.op0; oi < st.op0 + st.nops; oi++) {
-            const BOpD op = A.ops[oi];
-            switch (op.kind) {
-            case BK_U1: {
-                const V m0 = coef<V>(A.coef, op.coef), m1 = coef<V>(A.coef, op.coef + 1),
-                        m2 = coef<V>(A.coef, op.coef + 2), m3 = coef<V>(A.coef, op.coef + 3);
-                const uint32_t p = op.p0;
-                for (uint32_t j = tid; j < NL / 2; j += BT) {
-                    const uint32_t x0 = ins0(j, p), x1 = x0 | (1u << p);
-                    const V a = tile[x0], b = tile[x1];
-                    tile[x0] = cadd(cm(m0, a), cm(m1, b));
-                    tile[x1] = cadd(cm(m2, a), cm(m3, b));
+            for (uint32_t oi = op0; oi < op1; oi++) {
+                const BOpD op = ops[oi];
+                switch (op.kind) {
+                case BK_U1: {
+                    const V m0 = coef<V>(cf, op.coef), m1 = coef<V>(cf, op.coef + 1), m2 = coef<V>(cf, op.coef + 2),
+                            m3 = coef<V>(cf, op.coef + 3);
+                    const uint32_t sb = sw<V>(1u << op.p0);
+                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                        const uint32_t s1 = s0 ^ sb;
+                        const V a = tile[s0], b = tile[s1];
+                        tile[s0] = cadd(cm(m0, a), cm(m1, b));
+                        tile[s1] = cadd(cm(m2, a), cm(m3, b));
+                    });
+                    break;
                 }
-                break;
-            }
-            case BK_U1R: {
-                const T r0 = (T)A.coef[2 * op.coef], r1 = (T)A.coef[2 * op.coef + 2], r2 = (T)A.coef[2 * op.coef + 4],
-                        r3 = (T)A.coef[2 * op.coef + 6];
-                const uint32_t p = op.p0;
-                for (uint32_t j = tid; j < NL / 2; j += BT) {
-                    const uint32_t x0 = ins0(j, p), x1 = x0 | (1u << p);
-                    const V a = tile[x0], b = tile[x1];
-                    V u, w;
-                    u.x = r0 * a.x + r1 * b.x;
-                    u.y = r0 * a.y + r1 * b.y;
-                    w.x = r2 * a.x + r3 * b.x;
-                    w.y = r2 * a.y + r3 * b.y;
-                    tile[x0] = u;
-                    tile[x1] = w;
+                case BK_U1R: {
+                    const T r0 = (T)cf[2 * op.coef], r1 = (T)cf[2 * op.coef + 2], r2 = (T)cf[2 * op.coef + 4],
+                            r3 = (T)cf[2 * op.coef + 6];
+                    const uint32_t sb = sw<V>(1u << op.p0);
+                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                        const uint32_t s1 = s0 ^ sb;
+                        const V a = tile[s0], b = tile[s1];
+                        V u, w;
+                        u.x = r0 * a.x + r1 * b.x;
+                        u.y = r0 * a.y + r1 * b.y;
+                        w.x = r2 * a.x + r3 * b.x;
+                        w.y = r2 * a.y + r3 * b.y;
+                        tile[s0] = u;
+                        tile[s1] = w;
+                    });
+                    break;
                 }
-                break;
-            }
-            case BK_U1X: {
-                // [[d0, i o1], [i o2, d1]]
-                const T d0 = (T)A.coef[2 * op.coef], o1 = (T)A.coef[2 * op.coef + 3], o2 = (T)A.coef[2 * op.coef + 5],
-                        d1 = (T)A.coef[2 * op.coef + 6];
-                const uint32_t p = op.p0;
-                for (uint32_t j = tid; j < NL / 2; j += BT) {
-                    const uint32_t x0 = ins0(j, p), x1 = x0 | (1u << p);
-                    const V a = tile[x0], b = tile[x1];
-                    V u, w;
-                    u.x = d0 * a.x - o1 * b.y;
-                    u.y = d0 * a.y + o1 * b.x;
-                    w.x = d1 * b.x - o2 * a.y;
-                    w.y = d1 * b.y + o2 * a.x;
-                    tile[x0] = u;
-                    tile[x1] = w;
+                case BK_U1X: {
+                    // [[d0, i o1], [i o2, d1]]
+                    const T d0 = (T)cf[2 * op.coef], o1 = (T)cf[2 * op.coef + 3], o2 = (T)cf[2 * op.coef + 5],
+                            d1 = (T)cf[2 * op.coef + 6];
+                    const uint32_t sb = sw<V>(1u << op.p0);
+                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                        const uint32_t s1 = s0 ^ sb;
+                        const V a = tile[s0], b = tile[s1];
+                        V u, w;
+                        u.x = d0 * a.x - o1 * b.y;
+                        u.y = d0 * a.y + o1 * b.x;
+                        w.x = d1 * b.x - o2 * a.y;
+                        w.y = d1 * b.y + o2 * a.x;
+                        tile[s0] = u;
+                        tile[s1] = w;
+                    });
+                    break;
                 }
-                break;
-            }
-            case BK_U2: {
-                V m[16];
+                case BK_U2: {
+                    const uint32_t b0 = 1u << op.p0, b1 = 1u << op.p1;
+                    const uint32_t c1 = sw<V>(b0), c2 = sw<V>(b1), c3 = c1 ^ c2;
+                    visit<V>(NL, tid, b0 | b1, 0, [&](uint32_t, uint32_t s) {
+                        const uint32_t xs[4] = {s, s ^ c1, s ^ c2, s ^ c3};
+                        V v[4];
 #pragma unroll
-                for (int t = 0; t < 16; t++) m[t] = coef<V>(A.coef, op.coef + t);
-                const uint32_t p0 = op.p0, p1 = op.p1;
-                for (uint32_t j = tid; j < NL / 4; j += BT) {
-                    const uint32_t x = ins0(ins0(j, p0), p1);
-                    const uint32_t xs[4] = {x, x | (1u << p0), x | (1u << p1), x | (1u << p0) | (1u << p1)};
-                    V v[4];
+                        for (int t = 0; t < 4; t++) v[t] = tile[xs[t]];
 #pragma unroll
-                    for (int t = 0; t < 4; t++) v[t] = tile[xs[t]];
+                        for (int r2 = 0; r2 < 4; r2++) {
+                            V acc = cm(coef<V>(cf, op.coef + 4 * r2), v[0]);
 #pragma unroll
-                    for (int r2 = 0; r2 < 4; r2++) {
-                        V s = cm(m[4 * r2], v[0]);
+                            for (int c = 1; c < 4; c++) acc = cadd(acc, cm(coef<V>(cf, op.coef + 4 * r2 + c), v[c]));
+                            tile[xs[r2]] = acc;
+                        }
+                    });
+                    break;
+                }
+                case BK_G3: {
+                    // K unconditional 2x2s on distinct positions, 2^K amplitudes at a time
+                    const uint32_t K = op.k1;
+                    const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, K == 3 ? 1u << op.p2 : 0u};
+                    const uint32_t sp[3] = {sw<V>(pb[0]), sw<V>(pb[1]), sw<V>(pb[2])};
+                    V m[3][4];
 #pragma unroll
-                        for (int c2 = 1; c2 < 4; c2++) s = cadd(s, cm(m[4 * r2 + c2], v[c2]));
-                        tile[xs[r2]] = s;
+                    for (uint32_t i = 0; i < 3; i++)
+#pragma unroll
+                        for (uint32_t t = 0; t < 4; t++) m[i][t] = coef<V>(cf, op.coef + 4 * (i < K ? i : 0) + t);
+                    auto body = [&](uint32_t s, auto kk) {
+                        constexpr uint32_t KK = decltype(kk)::value, NA = 1u << KK;
+                        uint32_t xs[NA];
+                        V v[NA];
+#pragma unroll
+                        for (uint32_t a = 0; a < NA; a++) {
+                            uint32_t q = s;
+#pragma unroll
+                            for (uint32_t i = 0; i < KK; i++)
+                                if ((a >> i) & 1) q ^= sp[i];
+                            xs[a] = q;
+                            v[a] = tile[q];
+                        }
+#pragma unroll
+                        for (uint32_t i = 0; i < KK; i++) {
+#pragma unroll
+                            for (uint32_t a = 0; a < NA; a++) {
+                                if ((a >> i) & 1) continue;
+                                const V u = v[a], w = v[a | (1u << i)];
+                                v[a] = cadd(cm(m[i][0], u), cm(m[i][1], w));
+                                v[a | (1u << i)] = cadd(cm(m[i][2], u), cm(m[i][3], w));
+                            }
+                        }
+#pragma unroll
+                        for (uint32_t a = 0; a < NA; a++) tile[xs[a]] = v[a];
+                    };
+                    if (K == 3)
+                        visit<V>(NL, tid, pb[0] | pb[1] | pb[2], 0, [&](uint32_t, uint32_t s) { body(s, KTag<3>{}); });
+                    else
+                        visit<V>(NL, tid, pb[0] | pb[1], 0, [&](uint32_t, uint32_t s) { body(s, KTag<2>{}); });
+                    break;
+                }
+                case BK_DM: {
+                    if ((base & op.omask) != op.omask) break;
+                    const V ph = coef<V>(cf, op.coef);
+                    visit<V>(NL, tid, op.lmask, op.lmask, [&](uint32_t, uint32_t s) { tile[s] = cm(ph, tile[s]); });
+                    break;
+                }
+                case BK_PERM1: {
+                    if ((base & op.omask) != op.omask) break;
+                    const uint32_t bp = 1u << op.p0, sb = sw<V>(bp);
+                    const uint32_t k0 = op.k0, k1 = op.k1;
+                    visit<V>(NL, tid, op.lmask | bp, op.lmask, [&](uint32_t, uint32_t s0) {
+                        const uint32_t s1 = s0 ^ sb;
+                        const V a = tile[s0], b = tile[s1];
+                        tile[s1] = ipow(k0, a);
+                        tile[s0] = ipow(k1, b);
+                    });
+                    break;
+                }
+                case BK_SWAP: {
+                    const uint32_t b0 = 1u << op.p0, b1 = 1u << op.p1, sb = sw<V>(b0 | b1);
+                    visit<V>(NL, tid, b0 | b1, b0, [&](uint32_t, uint32_t s) {
+                        const uint32_t s2 = s ^ sb;
+                        const V a = tile[s], b = tile[s2];
+                        tile[s] = b;
+                        tile[s2] = a;
+                    });
+                    break;
+                }
+                case BK_NSLOT: {
+                    const uint32_t code = errs[en.err0 + op.a];
+                    if (!code) break;
+                    const uint32_t sb = sw<V>(1u << op.p0);
+                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                        const uint32_t s1 = s0 ^ sb;
+                        V a = tile[s0], b = tile[s1];
+                        if (code & 2) {
+                            b.x = -b.x;
+                            b.y = -b.y;
+                        }
+                        if (code & 1) {
+                            const V t = a;
+                            a = b;
+                            b = t;
+                        }
+                        tile[s0] = a;
+                        tile[s1] = b;
+                    });
+                    break;
+                }
+                case BK_MEASURE:
+                case BK_REDUCE:
+                case BK_COLLAPSE: {
+                    const BMGroupD &G = mgroups[op.c];
+                    const uint32_t D = G.D, NC = 1u << D;
+                    uint32_t ocell = 0;
+                    for (uint32_t d = 0; d < D; d++) ocell |= par64(base & G.om[d]) << d;
+                    auto cell = [&](uint32_t x) {
+                        uint32_t c = ocell;
+                        for (uint32_t d = 0; d < D; d++) c ^= par32(x & G.lm[d]) << d;
+                        return c;
+                    };
+                    double v[CMAX];
+#pragma unroll
+                    for (uint32_t c = 0; c < CMAX; c++) v[c] = 0.0;
+                    if (op.kind == BK_COLLAPSE) {
+                        // the cell weights from every tile's partials, same order in each tile
+                        const double *pp = A.part + en.part0 + op.a;
+                        for (uint32_t t = tid; t < op.b; t += BT) {
+#pragma unroll
+                            for (uint32_t c = 0; c < 16; c++)
+                                if (c < NC) v[c] += pp[NC * t + c];
+                        }
+                    } else {
+                        visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                            const V a = tile[s];
+                            const double w = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+                            const uint32_t cx = cell(x);
+#pragma unroll
+                            for (uint32_t c = 0; c < 16; c++) v[c] += (c == cx) ? w : 0.0;
+                        });
                     }
-                }
-                break;
-            }
-            case BK_DM: {
-                if ((base & op.omask) != op.omask) break;
-                const V ph = coef<V>(A.coef, op.coef);
-                const uint32_t cnt = NL >> __builtin_popcount(op.lmask);
-                for (uint32_t j = tid; j < cnt; j += BT) {
-                    const uint32_t x = deposit(j, op.lmask, op.lmask);
-                    tile[x] = cm(ph, tile[x]);
-                }
-                break;
-            }
-            case BK_PERM1: {
-                if ((base & op.omask) != op.omask) break;
-                const uint32_t p = op.p0;
-                const uint32_t cnt = NL >> (1 + __builtin_popcount(op.lmask));
-                for (uint32_t j = tid; j < cnt; j += BT) {
-                    const uint32_t x0 = deposit(j, op.lmask | (1u << p), op.lmask), x1 = x0 | (1u << p);
-                    const V a = tile[x0], b = tile[x1];
-                    tile[x1] = ipow(op.k0, a);
-                    tile[x0] = ipow(op.k1, b);
-                }
-                break;
-            }
-            case BK_SWAP: {
-                const uint32_t b0 = 1u << op.p0, b1 = 1u << op.p1;
-                for (uint32_t j = tid; j < NL / 4; j += BT) {
-                    const uint32_t x = deposit(j, b0 | b1, b0), y = x ^ (b0 | b1);
-                    const V a = tile[x], b = tile[y];
-                    tile[x] = b;
-                    tile[y] = a;
-                }
-                break;
-            }
-            case BK_NSLOT: {
-                const uint32_t code = A.errs[en.err0 + op.a];
-                if (!code) break;
-                const uint32_t p = op.p0;
-                for (uint32_t j = tid; j < NL / 2; j += BT) {
-                    const uint32_t x0 = ins0(j, p), x1 = x0 | (1u << p);
-                    V a = tile[x0], b = tile[x1];
-                    if (code & 2) {
-                        b.x = -b.x;
-                        b.y = -b.y;
+                    block_sum(v, NC, red, bc);
+                    if (op.kind == BK_REDUCE) {
+                        if (tid < NC) A.part[en.part0 + op.a + (uint64_t)NC * tileno + tid] = bc[tid];
+                        break;
                     }
-                    if (code & 1) {
-                        const V t = a;
-                        a = b;
-                        b = t;
+                    double W[16];
+#pragma unroll
+                    for (uint32_t c = 0; c < 16; c++) W[c] = c < NC ? bc[c] : 0.0;
+                    // the events in order, each restricting the kept cells (the
+                    // same arithmetic in every thread and every tile)
+                    uint32_t allow = (1u << NC) - 1, flip = 0;
+                    double scale = 1.0, lfac = 1.0;
+                    for (uint32_t e = 0; e < G.E; e++) {
+                        const uint32_t ed = G.ev[e].d, es = G.ev[e].sign, rp = G.ev[e].rpos;
+                        double nn = 0.0, ee = 0.0;
+#pragma unroll
+                        for (uint32_t c = 0; c < 16; c++) {
+                            const double w = ((allow >> c) & 1) ? W[c] : 0.0;
+                            nn += w;
+                            ee += (((c >> ed) & 1) ^ es) ? -w : w;
+                        }
+                        const uint32_t ord = en.m0 + G.ev[e].ord;
+                        const uint32_t o = decide(uni, forced, ord, nn, ee);
+#pragma unroll
+                        for (uint32_t c = 0; c < 16; c++)
+                            if ((((c >> ed) & 1) ^ es) != o) allow &= ~(1u << c);
+                        // complex64: renormalize small branches, the scale kept in FP64 per entry
+                        const double kept = 0.5 * (nn + (o ? -ee : ee)) * scale * scale;
+                        if (sizeof(T) == 4 && kept > 0.0 && kept < 0.25) {
+                            scale /= sqrt(kept);
+                            lfac *= sqrt(kept);
+                        }
+                        if (tid == 0 && tileno == 0) A.outc[ord] = (uint8_t)o;
+                        if (rp != 0xff && o) flip ^= 1u << rp;
                     }
-                    tile[x0] = a;
-                    tile[x1] = b;
-                }
-                break;
-            }
-            case BK_MEASURE:
-            case BK_REDUCE:
-            case BK_COLLAPSE: {
-                const uint32_t sgn = op.k0 ^ par64(base & op.omask);
-                double v[CMAX];
-                if (op.kind == BK_COLLAPSE) {
-                    // the measured weights from every tile's partials, same order in each tile
-                    double a0 = 0.0, a1 = 0.0;
-                    const double *pp = A.part + en.part0 + op.a;
-                    for (uint32_t t = tid; t < op.b; t += BT) {
-                        a0 += pp[2 * t];
-                        a1 += pp[2 * t + 1];
-                    }
-                    v[0] = a0;
-                    v[1] = a1;
-                } else {
-                    double a0 = 0.0, a1 = 0.0;
-                    for (uint32_t x = tid; x < NL; x += BT) {
-                        const V a = tile[x];
-                        const double w = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-                        a0 += w;
-                        a1 += (par32(x & op.lmask) ^ sgn) ? -w : w;
-                    }
-                    v[0] = a0;
-                    v[1] = a1;
-                }
-                block_sum(v, 2, red, bc);
-                const double snn = bc[0], se = bc[1];
-                if (op.kind == BK_REDUCE) {
-                    if (tid == 0) {
-                        A.part[en.part0 + op.a + 2 * tileno] = snn;
-                        A.part[en.part0 + op.a + 2 * tileno + 1] = se;
+                    if (tid == 0 && tileno == 0 && lfac != 1.0) A.lsc[rec.entry] *= lfac;
+                    const T sc = (T)scale;
+                    visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                        V a = tile[s];
+                        if (!((allow >> cell(x)) & 1)) {
+                            a.x = (T)0;
+                            a.y = (T)0;
+                        } else if (scale != 1.0) {
+                            a.x *= sc;
+                            a.y *= sc;
+                        }
+                        tile[s] = a;
+                    });
+                    if (flip) {
+                        // resets with outcome 1: x -> x ^ flip
+                        NCSV_SYNC();
+                        const uint32_t sf = sw<V>(flip);
+                        visit<V>(NL, tid, flip & (0u - flip), 0, [&](uint32_t, uint32_t s) {
+                            const uint32_t s2 = s ^ sf;
+                            const V a = tile[s], b = tile[s2];
+                            tile[s] = b;
+                            tile[s2] = a;
+                        });
                     }
                     break;
                 }
-                const uint32_t ord = en.m0 + op.c;
-                const uint32_t o = decide(A, ord, snn, se);
-                const double kept = 0.5 * (snn + (o ? -se : se));
-                // complex64: renormalize small branches, the scale kept in FP64 per entry
-                double sc = 1.0;
-                if (sizeof(T) == 4 && kept > 0.0 && kept < 0.25) sc = 1.0 / sqrt(kept);
-                if (tid == 0 && tileno == 0) {
-                    A.outc[ord] = (uint8_t)o;
-                    if (sc != 1.0) A.lsc[rec.entry] *= sqrt(kept);
-                }
-                const T s = (T)sc;
-                if (op.p0 == 0xff) {
-                    for (uint32_t x = tid; x < NL; x += BT) {
-                        V a = tile[x];
-                        if ((par32(x & op.lmask) ^ sgn) != o) {
-                            a.x = (T)0;
-                            a.y = (T)0;
-                        } else if (sc != 1.0) {
-                            a.x *= s;
-                            a.y *= s;
-                        }
-                        tile[x] = a;
-                    }
-                } else {
-                    // reset: keep the outcome's half and move it to bit p0 = 0
-                    const uint32_t p = op.p0;
-                    for (uint32_t j = tid; j < NL / 2; j += BT) {
-                        const uint32_t x0 = ins0(j, p), x1 = x0 | (1u << p);
-                        V a = o ? tile[x1] : tile[x0];
-                        if (sc != 1.0) {
-                            a.x *= s;
-                            a.y *= s;
-                        }
-                        V z;
-                        z.x = (T)0;
-                        z.y = (T)0;
-                        tile[x0] = a;
-                        tile[x1] = z;
-                    }
-                }
-                break;
-            }
-            case BK_EXPECT: {
-                const uint32_t K = op.b;
-                double v[CMAX];
+                case BK_EXPECT: {
+                    const uint32_t K = op.b;
+                    double v[CMAX];
 #pragma unroll
-                for (uint32_t k = 0; k < CMAX; k++) v[k] = 0.0;
-                for (uint32_t x = tid; x < NL; x += BT) {
-                    const V a = tile[x];
-                    const double ax = (double)a.x, ay = (double)a.y;
+                    for (uint32_t k = 0; k < CMAX; k++) v[k] = 0.0;
+                    visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                        const V a = tile[s];
+                        const double ax = (double)a.x, ay = (double)a.y;
 #pragma unroll
-                    for (uint32_t k = 0; k < CMAX / 2; k++) {
-                        if (k < K) {
-                            const BPauliD P = A.paulis[op.c + k];
-                            const V b = tile[x ^ P.xl];
-                            const double bx = (double)b.x, by = (double)b.y;
-                            // conj(b) a
-                            double re = bx * ax + by * ay, im = bx * ay - by * ax;
-                            const uint32_t pw = (P.ph + 2 * (par32(P.zl & x) ^ par64(P.zo & base))) & 3;
-                            double tr = re, ti = im;
-                            if (pw == 1) {
-                                tr = -im;
-                                ti = re;
-                            } else if (pw == 2) {
-                                tr = -re;
-                                ti = -im;
-                            } else if (pw == 3) {
-                                tr = im;
-                                ti = -re;
+                        for (uint32_t k = 0; k < CMAX / 2; k++) {
+                            if (k < K) {
+                                const BPauliD P = paulis[op.c + k];
+                                const V b = tile[s ^ sw<V>(P.xl)];
+                                const double bx = (double)b.x, by = (double)b.y;
+                                // conj(b) a
+                                const double re = bx * ax + by * ay, im = bx * ay - by * ax;
+                                const uint32_t pw = (P.ph + 2 * (par32(P.zl & x) ^ par64(P.zo & base))) & 3;
+                                double tr = re, ti = im;
+                                if (pw == 1) {
+                                    tr = -im;
+                                    ti = re;
+                                } else if (pw == 2) {
+                                    tr = -re;
+                                    ti = -im;
+                                } else if (pw == 3) {
+                                    tr = im;
+                                    ti = -re;
+                                }
+                                v[2 * k] += tr;
+                                v[2 * k + 1] += ti;
                             }
-                            v[2 * k] += tr;
-                            v[2 * k + 1] += ti;
                         }
+                    });
+                    block_sum(v, 2 * K, red, bc);
+                    if (tid < 2 * K) A.pexp[en.exp0 + op.a + (uint64_t)2 * K * tileno + tid] = bc[tid];
+                    break;
+                }
+                default: break;
+                }
+                NCSV_SYNC();
+            }
+
+            if (st.flags & BF_STORE) {
+                uint64_t tb = base + offlo;
+                NCSV_OPAQUE(tb);  // recomputed here, not held live across the ops
+#pragma unroll
+                for (uint32_t k = 0; k < KH; k++) {
+                    if (k < kcnt) {
+                        uint64_t off = tb;
+#pragma unroll
+                        for (uint32_t b = 0; b < HB; b++)
+                            if ((k >> b) & 1) off += wq[b];
+                        g[off] = tile[st_tid ^ sw<V>(BT * k)];
                     }
                 }
-                block_sum(v, 2 * K, red, bc);
-                if (tid < 2 * K) A.pexp[en.exp0 + op.a + (uint64_t)2 * K * tileno + tid] = bc[tid];
-                break;
             }
-            default: break;
-            }
+        }  // tiles of the item
+        if (rec.flags & RF_RELEASE) {
+            NCSV_FENCE_RELEASE();
+            NCSV_SYNC();
+            if (wave == 0) NCSV_ADD_AGENT(A.done + r, tid == 0 ? ntl : 0u);
+        } else {
             NCSV_SYNC();
         }
-
-        if (st.flags & BF_STORE) {
-            uint32_t k = 0;
-            for (uint32_t x = tid; x < NL; x += BT, k++) {
-                uint64_t off = offlo;
-                for (uint32_t b = 0; b + LOB < L; b++)
-                    if ((k >> b) & 1) off |= 1ull << qt[LOB + b];
-                g[base + off] = tile[x];
-            }
-        }
-        NCSV_FENCE_RELEASE();
-        NCSV_SYNC();
-        if (wave == 0) NCSV_ADD_AGENT(A.done + r, tid == 0 ? 1u : 0u);
     }
 }
 

@@ -75,6 +75,7 @@ double shfl_xor(double x, unsigned mask) {
 #define NCSV_SLEEP() std::this_thread::yield()
 #define NCSV_UNIFORM64(v) (v)
 #define NCSV_UNIFORM32(v) (v)
+#define NCSV_OPAQUE(x) ((void)0)
 #define NCSV_LANE0_STORE(dst, v) \
     do {                         \
         if (NCSV_TID == 0) (dst) = (v); \
@@ -97,9 +98,11 @@ void simulate(const BArgs &A, ncsv_precision prec) {
         th.emplace_back([&A, prec, t]() {
             sim::tid = t;
             if (prec == NCSV_C128)
-                kern::k_batch<double, 12>(A);
+                kern::k_batch<double, 12>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.qtab, A.entries, A.recs, A.rec_blk,
+                                          A.uni, A.forced, A.errs, A);
             else
-                kern::k_batch<float, 13>(A);
+                kern::k_batch<float, 13>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.qtab, A.entries, A.recs, A.rec_blk,
+                                         A.uni, A.forced, A.errs, A);
         });
     }
     for (auto &x : th) x.join();

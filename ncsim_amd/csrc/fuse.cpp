@@ -226,6 +226,15 @@ int reg_bit(uint32_t q, const uint32_t *pos_of, uint32_t rpos) {
 // matrices with real entries, or with a real diagonal and an imaginary
 // off-diagonal, get their cheaper forms.
 void finish_sweep(SweepH *sw) {
+    // 2x2s left diagonal by merging go back to the diagonal form
+    for (TileOpH &o : sw->ops) {
+        if (o.kind == TK_U1 && o.m[1] == cd(0, 0) && o.m[2] == cd(0, 0)) {
+            o.kind = TK_D1R;
+            o.m[1] = o.m[3];
+            o.rmask = 0;
+            o.gmask = 0;
+        }
+    }
     cd gphase(1.0, 0.0);
     for (TileOpH &o : sw->ops) {
         if (o.kind == TK_D1R || o.kind == TK_D1T) {
@@ -365,6 +374,7 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
             case TK_U1: {
                 o.j0 = (uint8_t)reg_bit(g.q0, pos_of.data(), rpos);
                 for (int t = 0; t < 4; t++) o.m[t] = g.m[t];
+            merge_u1:
                 const int l = last[g.q0];
                 if (l >= 0 && sw.ops[l].kind == TK_U1 && sw.ops[l].j0 == o.j0) {
                     // merge: later x earlier (FP64 on the host)
@@ -404,7 +414,15 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
                 o.m[0] = g.m[0];
                 o.m[1] = g.m[1];
                 if (b >= 0) {
+                    // a one-qubit diagonal on a register bit: a 2x2 that merges
+                    // with its neighbours (split again by the peephole pass)
+                    o.kind = TK_U1;
                     o.j0 = (uint8_t)b;
+                    o.m[0] = g.m[0];
+                    o.m[1] = cd(0, 0);
+                    o.m[2] = cd(0, 0);
+                    o.m[3] = g.m[1];
+                    goto merge_u1;
                 } else {
                     o.kind = TK_D1T;
                     o.gmask = 1ull << g.q0;
@@ -414,6 +432,18 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
             case TK_DM:
                 split(g.cmask);
                 o.m[0] = g.m[0];
+                if (o.gmask == 0 && __builtin_popcount(o.rmask) == 1 && g.cmask == g.sup &&
+                    __builtin_popcountll(g.sup) == 1) {
+                    // phase on one register bit: a diagonal 2x2, as above
+                    o.kind = TK_U1;
+                    o.j0 = (uint8_t)__builtin_ctz(o.rmask);
+                    o.rmask = 0;
+                    o.m[0] = cd(1, 0);
+                    o.m[1] = cd(0, 0);
+                    o.m[2] = cd(0, 0);
+                    o.m[3] = g.m[0];
+                    goto merge_u1;
+                }
                 break;
             case TK_PERM1:
                 o.j0 = (uint8_t)reg_bit(g.q0, pos_of.data(), rpos);

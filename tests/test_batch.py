@@ -185,6 +185,32 @@ def test_emulated_batch_seeds_and_forced():
         assert ex == first[i][0] and nn == first[i][1]
 
 
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+@pytest.mark.parametrize("tile_bits", [0, 6])
+def test_measure_reset_runs_match_oracle(precision, tile_bits):
+    """Runs of M and R on several qubits (decided from one joint reduction)
+    in noisy brickwork trajectories, emulated, against the oracle run with
+    the same draws."""
+    from ncsim_amd import workloads as W
+    text = W.noisy_brickwork(8, 12, 4, p=0.05, every=4, mqubits=[0, 2, 5, 7])
+    c = nc.Circuit.parse(text)
+    ps = [OPauli(8, 0, 1 << i, 0) for i in range(8)] + [OPauli(8, 0, 3 << i, 0) for i in range(7)]
+    b = nc.HipBatch(precision)
+    if tile_bits:
+        b.set_tile_bits(tile_bits)
+    for _ in range(6):
+        b.add(c, ps)
+    b.emulate(seeds=list(range(6)))
+    for i in range(6):
+        g = OS.SplitMix64(i)
+        us = [g.gen_f64() for _ in range(b.draws(i))]
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(text, ps, us)
+        assert oc == roc, f"entry {i}"
+        assert abs(nn - rnn) < 1e-12
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+
+
 def test_batch_programs_share_and_count():
     """Entries running one circuit share its program; stats count them."""
     text = random_circuit(8, 5, 9, measure=True, noise=True)
