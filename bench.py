@@ -27,6 +27,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(n, depth, fusion, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    runs of this exact workload (profiles/traffic_*.json), else None."""
+    path = os.path.join(ROOT, "profiles", "traffic_sweep_c128_brickwork30.json")
+    if (n, depth, fusion) != (30, 24, 1) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["bytes_per_launch"] if t.get("kernel") == kernel else None
+
+
 def cpu_baseline(seconds=15.0):
     """The oracle (C restatement of StateVector<C64>, OpenMP) applying the
     26-qubit member of the same circuit family gate by gate until `seconds`
@@ -133,12 +144,20 @@ def run_batch(args):
     gates = np.array([W.count_gates(t) for _, t, _ in specs], dtype=np.float64)
     ns = np.array([n for n, _, _ in specs])
 
+    tuned = {}
+
     def batched(idx):
         b = H.HipBatch(prec)
-        if args.tile_bits:
-            b.set_tile_bits(args.tile_bits)
         for i in idx:
             b.add(circs[i], paulis[i])
+        if args.tile_bits:
+            b.set_tile_bits(args.tile_bits)
+        elif not args.no_autotune:
+            t0 = time.perf_counter()
+            r = b.autotune(log=log)
+            tuned[len(idx)] = {"config": r["config"], "cached": r["cached"],
+                               "seconds": time.perf_counter() - t0}
+            log(f"autotune: {r['config']} ({'cached' if r['cached'] else 'measured'})")
         t0 = time.perf_counter()
         b.run()  # plans every circuit and uploads the programs
         cold = time.perf_counter() - t0
@@ -202,6 +221,7 @@ def run_batch(args):
                            "ms_per_circuit": per * 1000,
                            "path": "single-state fused interpreter, one state per qubit count"},
            "speedup_vs_per_circuit": cps * per, "max_abs_batch_vs_single": max(maxdiff, maxdiff_s),
+           "autotune": tuned,
            "small_circuits": {"qubits": "10-14", "count": len(small), "circuits_per_s": len(small) / wall_s,
                               "kernel_ms": kms_s, "per_circuit_circuits_per_s": 1.0 / per_s,
                               "speedup_vs_per_circuit": len(small) / wall_s * per_s}}
@@ -306,6 +326,7 @@ def main():
     ap.add_argument("--single", type=int, default=3, help="trajectories timed through per-trajectory launches")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the state-vector path runs on a single "
@@ -366,7 +387,9 @@ def main():
     achieved = byts / launches / avg_s
     log("bench: kernels " + ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(stats.items())))
     roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "launches": launches,
+                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(n, depth, args.fusion, name),
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                "algorithmic_bytes": byts / launches, "launches": launches,
                 "avg_ms": tot_ms / launches, "sweeps_per_step": sweeps}
     cpu = None if args.no_cpu else cpu_baseline()
     out = {

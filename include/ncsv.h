@@ -88,6 +88,9 @@ typedef double (*ncsv_uniform_fn)(void *ctx);
 size_t ncsv_last_error(char *buf, size_t cap);
 /* Number of visible GPUs (0 without a device; never fails). */
 int ncsv_device_count(void);
+/* The current device's architecture and CU count, e.g. "gfx950:sramecc+:xnack- 256cu"
+ * (keys tuning caches; Device without a GPU). */
+ncsv_status ncsv_device_arch(char *buf, size_t cap);
 
 /* ---- lifetime and construction (vector.rs:24-64) ------------------------ */
 
@@ -279,6 +282,9 @@ ncsv_status ncsv_batch_draws(const ncsv_batch *b, uint32_t entry, uint32_t *coun
  * the persistent grid (0 = every resident workgroup). */
 ncsv_status ncsv_batch_set_tile_bits(ncsv_batch *b, uint32_t bits);
 ncsv_status ncsv_batch_set_blocks(ncsv_batch *b, int32_t blocks);
+/* The tile's fixed low qubits (coalescing against fewer sweeps; -1 = 5 for
+ * complex64, 4 for complex128) and the most tiles per queue item (-1 = 8). */
+ncsv_status ncsv_batch_set_shape(ncsv_batch *b, int32_t low_bits, int32_t tiles_per_item);
 /* Runs every entry.  `uniforms` holds each entry's draws back to back (in
  * entry order), or is NULL and entry e's draws come from a SplitMix64 stream
  * seeded with seeds[e] (gen_f64 = (next >> 11) * 2^-53).  `forced`, laid out

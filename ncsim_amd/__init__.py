@@ -454,10 +454,12 @@ class HipBatch:
     Reset and Noise op takes one uniform per run, in program order."""
 
     def __init__(self, precision=C128):
+        self.precision = precision
         self._h = L.lib().ncsv_batch_new(int(precision))
         if not self._h:
             _check(8)
         self._circuits = []  # kept alive with the batch
+        self._paulis = []
         self._npauli = []
         self._ndraws = []
 
@@ -479,6 +481,7 @@ class HipBatch:
         d = ctypes.c_uint32()
         _check(L.lib().ncsv_batch_draws(self._h, idx.value, ctypes.byref(d)))
         self._circuits.append(circuit)
+        self._paulis.append(list(paulis))
         self._npauli.append(k)
         self._ndraws.append(d.value)
         return idx.value
@@ -491,6 +494,16 @@ class HipBatch:
 
     def set_blocks(self, blocks):
         _check(L.lib().ncsv_batch_set_blocks(self._h, int(blocks)))
+
+    def set_shape(self, low_bits=-1, tiles_per_item=-1):
+        """Fixed low tile qubits and the most tiles per queue item (-1: defaults)."""
+        _check(L.lib().ncsv_batch_set_shape(self._h, int(low_bits), int(tiles_per_item)))
+
+    def autotune(self, **kw):
+        """Chooses tile width, low qubits and queue item size by timing
+        candidates on a sample of this batch (ncsim_amd.autotune)."""
+        from .autotune import tune_batch
+        return tune_batch(self, **kw)
 
     def _args(self, uniforms, seeds, forced):
         import numpy as np

@@ -77,12 +77,14 @@ _SIGS = {
     "ncsv_batch_draws": [ctypes.c_void_p, ctypes.c_uint32, c_u32p],
     "ncsv_batch_set_tile_bits": [ctypes.c_void_p, ctypes.c_uint32],
     "ncsv_batch_set_blocks": [ctypes.c_void_p, ctypes.c_int32],
+    "ncsv_batch_set_shape": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     "ncsv_batch_run": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
     "ncsv_batch_emulate": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
     "ncsv_batch_simulate": [ctypes.c_void_p, c_dp, c_u64p, c_u8p],
     "ncsv_batch_result": [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_u8p],
     "ncsv_batch_stats": [ctypes.c_void_p, c_dp],
     "ncsv_batch_plan": [ctypes.c_void_p],
+    "ncsv_device_arch": [ctypes.c_char_p, ctypes.c_size_t],
 }
 
 _lib = None

@@ -1,0 +1,114 @@
+"""Autotuner for the batched form.
+
+The batched kernel's speed depends on three planning choices that trade
+against each other differently per workload and device:
+
+- tile width (qubits per fused sweep, at most 13 for complex64 and 12 for
+  complex128): wider tiles cover more gates per pass over the state;
+- fixed low tile qubits (5 / 4 by default): more give longer contiguous
+  runs per load, fewer leave the planner more freedom (fewer sweeps);
+- tiles per queue item: longer items need fewer device-scope releases,
+  shorter ones spread a step over more workgroups.
+
+`tune_batch` times each candidate on a stratified sample of the batch's own
+entries (same circuits, same Paulis, seeded draws), applies the fastest to
+the batch and records it in a JSON cache keyed by device, precision and the
+batch's shape (qubit-count histogram, draws or none), so later batches of
+the same kind skip the search.
+"""
+import ctypes
+import json
+import os
+import time
+
+from . import _lib as L
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ncsim_amd", "batch_tune.json")
+
+
+def device_arch():
+    buf = ctypes.create_string_buffer(256)
+    st = L.lib().ncsv_device_arch(buf, 256)
+    if st != 0:
+        from . import _check
+        _check(st)
+    return buf.value.decode()
+
+
+def _signature(batch):
+    ns = [c.num_qubits() for c in batch._circuits]
+    buckets = {}
+    for n in ns:
+        b = f"{(n // 4) * 4}-{(n // 4) * 4 + 3}"
+        buckets[b] = buckets.get(b, 0) + 1
+    total = len(ns)
+    hist = ",".join(f"{k}:{round(v / total, 1)}" for k, v in sorted(buckets.items()))
+    noisy = "draws" if any(batch._ndraws) else "nodraws"
+    return f"{'c64' if batch.precision == 1 else 'c128'}|{hist}|{noisy}"
+
+
+def candidates(precision):
+    top = 13 if precision == 1 else 12
+    low = 5 if precision == 1 else 4
+    out = []
+    for bits in (top, top - 1):
+        for lo in (low, 3):
+            for items in (8, 32):
+                out.append({"tile_bits": bits, "low_bits": lo, "tiles_per_item": items})
+    return out
+
+
+def _apply(batch, cfg):
+    batch.set_tile_bits(cfg["tile_bits"])
+    batch.set_shape(cfg["low_bits"], cfg["tiles_per_item"])
+
+
+def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None):
+    """Chooses and applies the fastest candidate; returns
+    {"config": ..., "timings": [(config, kernel_ms), ...], "cached": bool}."""
+    from . import HipBatch
+
+    cache = cache or os.environ.get("NCSV_TUNE_CACHE", DEFAULT_CACHE)
+    key = f"{device_arch()}|{_signature(batch)}"
+    db = {}
+    if os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                db = json.load(f)
+        except (OSError, ValueError):
+            db = {}
+    if not force and key in db:
+        _apply(batch, db[key]["config"])
+        return {"config": db[key]["config"], "timings": db[key].get("timings", []), "cached": True}
+
+    n = len(batch)
+    order = sorted(range(n), key=lambda i: batch._circuits[i].num_qubits())
+    step = max(1, n // sample)
+    idx = order[::step][:sample]
+    seeds = list(range(len(idx)))
+    timings = []
+    for cfg in candidates(batch.precision):
+        b = HipBatch(batch.precision)
+        _apply(b, cfg)
+        for i in idx:
+            b.add(batch._circuits[i], batch._paulis[i])
+        b.run(seeds=seeds)  # plans and uploads
+        best = None
+        for _ in range(reps):
+            b.run(seeds=seeds)
+            ms = b.stats()["kernel_ms"]
+            best = ms if best is None else min(best, ms)
+        timings.append((cfg, best))
+        if log:
+            log(f"autotune {cfg}: {best:.2f} ms")
+        del b
+    cfg = min(timings, key=lambda t: t[1])[0]
+    _apply(batch, cfg)
+    db[key] = {"config": cfg, "timings": timings, "sample": len(idx), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    try:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(db, f, indent=1)
+    except OSError:
+        pass
+    return {"config": cfg, "timings": timings, "cached": False}

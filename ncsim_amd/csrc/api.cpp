@@ -177,6 +177,19 @@ size_t ncsv_last_error(char *buf, size_t cap) {
     return g_last_error.size();
 }
 
+ncsv_status ncsv_device_arch(char *buf, size_t cap) {
+    return guard([&] {
+        need(buf, "buffer");
+        int dev = 0;
+        check_hip(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t p;
+        check_hip(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+        const std::string a = std::string(p.gcnArchName) + " " + std::to_string(p.multiProcessorCount) + "cu";
+        if (cap == 0 || a.size() + 1 > cap) fail(NCSV_INVALID, "buffer too small");
+        memcpy(buf, a.c_str(), a.size() + 1);
+    });
+}
+
 int ncsv_device_count(void) {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) return 0;

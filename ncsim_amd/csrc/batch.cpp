@@ -314,7 +314,7 @@ std::vector<BOpD> group_ops(StepH *s, bool drop_slots) {
 // Plans one circuit (plus its Paulis) for tiles of at most Lb qubits.
 std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<uint64_t> &X,
                                        const std::vector<uint64_t> &Z, const std::vector<uint32_t> &PH,
-                                       ncsv_precision prec, uint32_t Lb) {
+                                       ncsv_precision prec, uint32_t Lb, int low_bits) {
     auto P = std::make_shared<Program>();
     const uint32_t n = c->n;
     P->n = n;
@@ -322,8 +322,7 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     P->L = L;
     // low qubits fixed in every gate tile (256-byte runs), leaving room for a
     // two-qubit gate plus a reset qubit on small tiles
-    uint32_t cl = std::min(prec == NCSV_C64 ? 5u : 4u, L > 3 ? L - 3 : 0u);
-    if (const char *e = getenv("NCSV_BATCH_LOW")) cl = std::min((uint32_t)atoi(e), cl);
+    uint32_t cl = std::min(low_bits >= 0 ? (uint32_t)low_bits : (prec == NCSV_C64 ? 5u : 4u), L > 3 ? L - 3 : 0u);
 
     // the circuit as gate runs and measurement events (ticks and annotations dropped)
     std::vector<Ev> evs;
@@ -618,6 +617,8 @@ struct ncsv_batch {
     ncsv_precision prec = NCSV_C128;
     uint32_t tile_bits = 0;  // 0: the largest tile for the precision
     int blocks = 0;          // 0: every resident workgroup
+    int low_bits = -1;       // fixed low tile qubits (-1: 5 for complex64, 4 for complex128)
+    int items_lg = -1;       // log2 of the most tiles per queue item (-1: 3)
     struct Entry {
         const ncsv_circuit *c = nullptr;
         std::vector<uint64_t> x, z;
@@ -628,6 +629,7 @@ struct ncsv_batch {
     std::vector<Entry> entries;
     std::vector<std::shared_ptr<Program>> progs;
     uint32_t planned_bits = 0;
+    int planned_low = -1;
     // device program tables
     void *d_tab = nullptr;
     size_t tab_bytes = 0;
@@ -675,7 +677,8 @@ uint32_t tile_bits_of(const ncsv_batch *b) {
 // same Paulis share one), in parallel host threads.
 void plan_all(ncsv_batch *b) {
     const uint32_t Lb = tile_bits_of(b);
-    if (Lb != b->planned_bits) {
+    if (Lb != b->planned_bits || b->low_bits != b->planned_low) {
+        b->planned_low = b->low_bits;
         for (auto &e : b->entries) e.prog = -1;
         b->progs.clear();
         b->planned_bits = Lb;
@@ -711,7 +714,7 @@ void plan_all(ncsv_batch *b) {
                 const size_t k = next.fetch_add(1);
                 if (k >= todo.size()) return;
                 const auto &e = b->entries[todo[k]];
-                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb);
+                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb, b->low_bits);
             }
         }));
     }
@@ -858,7 +861,7 @@ void fill_draws(const Program &P, const double *u, uint64_t *seed, const uint8_t
 Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *uniforms,
                  const std::vector<size_t> &draw_off, const uint64_t *seeds, const uint8_t *forced) {
     // tiles per queue item: fewer, longer items mean fewer device-scope releases
-    uint32_t lg_tiles_per_item = 3;
+    uint32_t lg_tiles_per_item = b->items_lg >= 0 ? (uint32_t)b->items_lg : 3u;
     if (const char *e = getenv("NCSV_BATCH_TILES_PER_ITEM")) {
         const int v = atoi(e);
         lg_tiles_per_item = 0;
@@ -1420,6 +1423,21 @@ ncsv_status ncsv_batch_set_tile_bits(ncsv_batch *b, uint32_t bits) {
             ncsv::fail(NCSV_INVALID, "tile bits out of range");
         std::lock_guard<std::mutex> lk(b->mu);
         b->tile_bits = bits;
+    });
+}
+
+ncsv_status ncsv_batch_set_shape(ncsv_batch *b, int32_t low_bits, int32_t tiles_per_item) {
+    return ncsv::guard_status([&]() {
+        if (!b) ncsv::fail(NCSV_INVALID, "null batch");
+        if (low_bits > 8 || tiles_per_item > 1024) ncsv::fail(NCSV_INVALID, "shape out of range");
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->low_bits = low_bits < 0 ? -1 : low_bits;
+        int lg = -1;
+        if (tiles_per_item > 0) {
+            lg = 0;
+            while ((2 << lg) <= tiles_per_item) lg++;
+        }
+        b->items_lg = lg;
     });
 }
 

@@ -254,6 +254,34 @@ def test_kernel_code_on_host_matches_oracle(precision):
             assert abs(a - r) < tol, (i, a, r)
 
 
+@pytest.mark.parametrize("low_bits,tiles", [(3, 1), (0, 2), (2, 32)])
+def test_emulated_shapes_match_oracle(low_bits, tiles):
+    """Other tile shapes (fewer fixed low qubits, queue item sizes) plan
+    programs that still equal the oracle."""
+    cases = CASES[2:]
+    texts = [random_circuit(n, d, s) for n, d, s in cases]
+    paulis = [random_paulis(n, 4, s) for n, d, s in cases]
+    b = _build(texts, paulis, nc.C128, 7)
+    b.set_shape(low_bits, tiles)
+    us = _draws(b, 5)
+    b.emulate(uniforms=us)
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+
+
+def test_autotune_candidates_and_signature():
+    from ncsim_amd import autotune as AT
+    c64 = AT.candidates(nc.C64)
+    assert {c["tile_bits"] for c in c64} == {13, 12}
+    assert all(c["low_bits"] in (5, 3) for c in c64)
+    b = _build([random_circuit(10, 3, 1), random_circuit(14, 3, 2)], [[], []], nc.C128, 0)
+    sig = AT._signature(b)
+    assert sig.startswith("c128|") and "draws" in sig
+
+
 def test_batch_rejects_unsupported():
     b = nc.HipBatch(nc.C128)
     c = nc.Circuit.parse("QUBITS 3\nH 0\nMX 1\n")
@@ -299,6 +327,27 @@ def test_gpu_batch_uneven_sizes_and_many_entries():
         assert oc == roc, f"entry {i}"
         assert max([0.0] + [abs(a - r) for a, r in zip(ex, rex)]) < 1e-12
         assert abs(nn - rnn) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_autotune_keeps_results(tmp_path):
+    """The autotuner times its candidates on a sample, applies one, caches
+    it; results stay those of the oracle."""
+    specs = [(n, 6, 300 + n) for n in (10, 12, 14, 15, 16)]
+    texts = [random_circuit(n, d, s, noise=False) for n, d, s in specs]
+    paulis = [random_paulis(n, 4, s) for n, d, s in specs]
+    b = _build(texts, paulis, nc.C128, 0)
+    cache = str(tmp_path / "tune.json")
+    r = b.autotune(sample=5, reps=1, cache=cache)
+    assert not r["cached"] and r["config"] in [c for c, _ in r["timings"]]
+    assert b.autotune(sample=5, reps=1, cache=cache)["cached"]
+    us = _draws(b, 3)
+    b.run(uniforms=us)
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = b.result(i)
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc
+        assert max(abs(a - r_) for a, r_ in zip(ex, rex)) < 1e-12
 
 
 @pytest.mark.gpu
