@@ -634,7 +634,9 @@ struct ncsv_batch {
     void *d_tab = nullptr;
     size_t tab_bytes = 0;
     size_t off_steps = 0, off_ops = 0, off_coef = 0, off_paulis = 0, off_mgroups = 0, off_qtab = 0;
-    bool tables_dirty = true;
+    bool tables_dirty = true;               // device copy out of date
+    std::shared_ptr<void> flat;             // host tables of the current programs
+    bool flat_dirty = true;
     // per-chunk device buffers (grow only)
     void *d_amp = nullptr;
     size_t amp_cap = 0;
@@ -683,6 +685,7 @@ void plan_all(ncsv_batch *b) {
         b->progs.clear();
         b->planned_bits = Lb;
         b->tables_dirty = true;
+        b->flat_dirty = true;
     }
     std::vector<size_t> todo;
     for (size_t i = 0; i < b->entries.size(); i++) {
@@ -726,6 +729,7 @@ void plan_all(ncsv_batch *b) {
     for (auto &e : b->entries)
         if (e.prog <= -2) e.prog = b->entries[(size_t)(-2 - e.prog)].prog;
     b->tables_dirty = true;
+    b->flat_dirty = true;
     b->plan_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1304,7 +1308,12 @@ void run_batch(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, con
     for (const auto &e : b->entries)
         if (e.ndraws && !uniforms && !seeds) fail(NCSV_INVALID, "batched run: circuits with draws need uniforms or seeds");
     plan_all(b);
-    const Tables t = flatten(b);
+    if (!b->flat || b->flat_dirty) {
+        b->flat = std::make_shared<Tables>(flatten(b));
+        b->flat_dirty = false;
+        b->tables_dirty = true;
+    }
+    const Tables &t = *std::static_pointer_cast<Tables>(b->flat);
     if (!emulate && b->tables_dirty) upload_tables(b, t);
     std::vector<size_t> draw_off(b->entries.size() + 1, 0);
     for (size_t e = 0; e < b->entries.size(); e++) draw_off[e + 1] = draw_off[e] + b->entries[e].ndraws;

@@ -413,7 +413,7 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
                 const int b = reg_bit(g.q0, pos_of.data(), rpos);
                 o.m[0] = g.m[0];
                 o.m[1] = g.m[1];
-                if (b >= 0) {
+                if (b >= 0 && geo.merge_diag) {
                     // a one-qubit diagonal on a register bit: a 2x2 that merges
                     // with its neighbours (split again by the peephole pass)
                     o.kind = TK_U1;
@@ -423,6 +423,8 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
                     o.m[2] = cd(0, 0);
                     o.m[3] = g.m[1];
                     goto merge_u1;
+                } else if (b >= 0) {
+                    o.j0 = (uint8_t)b;
                 } else {
                     o.kind = TK_D1T;
                     o.gmask = 1ull << g.q0;
@@ -432,7 +434,7 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
             case TK_DM:
                 split(g.cmask);
                 o.m[0] = g.m[0];
-                if (o.gmask == 0 && __builtin_popcount(o.rmask) == 1 && g.cmask == g.sup &&
+                if (geo.merge_diag && o.gmask == 0 && __builtin_popcount(o.rmask) == 1 && g.cmask == g.sup &&
                     __builtin_popcountll(g.sup) == 1) {
                     // phase on one register bit: a diagonal 2x2, as above
                     o.kind = TK_U1;
@@ -530,6 +532,7 @@ std::vector<SweepH> plan_lds_sweeps(const std::vector<LOp> &ops, uint32_t n, uin
     geo.LT = 0;
     geo.R = L;
     geo.c = c;
+    geo.merge_diag = true;
     std::vector<SweepH> out = plan_sweeps(cl, run, n, geo, first_fixed);
     if (out.empty() && first_fixed) {
         // no gates: an empty sweep whose tile holds the fixed qubits

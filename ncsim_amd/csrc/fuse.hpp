@@ -60,6 +60,8 @@ struct SegItem {
 struct TileGeom {
     uint32_t LT = 8, R = 4, c = 4;  // threads = 2^LT, registers per thread = 2^R, fixed low qubits
     bool pipe = false;              // load the next op while applying the current one
+    bool merge_diag = false;        // multiply one-qubit diagonals into neighbouring 2x2s (saves passes
+                                    // where ops cost a pass each; costs VALU where 2x2s have cheap forms)
     uint32_t L() const { return LT + R; }
     int key() const { return (int)((LT << 12) | (R << 8) | (c << 4) | (pipe ? 1 : 0)); }
 };

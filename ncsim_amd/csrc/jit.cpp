@@ -34,12 +34,32 @@ struct Gen {
 
     std::string var(uint32_t r) const { return "a" + std::to_string(name[r]); }
 
+    bool literal = false;  // coefficients as constants in the code instead of loads from c[]
+    bool dbl = true;
+
     uint32_t push(double v) {
         coef->push_back(v);
         return (uint32_t)coef->size() - 1;
     }
-    std::string C(uint32_t i) const { return "c[" + std::to_string(i) + "]"; }
+    std::string C(uint32_t i) const {
+        if (!literal) return "c[" + std::to_string(i) + "]";
+        // exact hexadecimal literal of the value the kernel would have loaded
+        std::ostringstream e;
+        e << std::hexfloat;
+        if (dbl)
+            e << "(" << (*coef)[i] << ")";
+        else
+            e << "(" << (double)(float)(*coef)[i] << "f)";
+        return e.str();
+    }
 };
+
+// Coefficients compiled in as constants (default; NCSV_JIT_LITERALS=0 loads
+// them from the coefficient buffer so that one kernel serves every angle).
+bool use_literals() {
+    const char *e = getenv("NCSV_JIT_LITERALS");
+    return !(e && atoi(e) == 0);
+}
 
 // Expression depositing bits 0..k-1 of `src` on the bit positions `dst`
 // (ascending source bits), grouping runs of consecutive destinations.
@@ -383,6 +403,8 @@ bool diag_run_pays(const std::vector<const TileOpH *> &run, uint32_t NR) {
 std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef) {
     Gen g;
     g.coef = coef;
+    g.literal = use_literals();
+    g.dbl = dbl;
     g.LT = geo.LT;
     g.R = geo.R;
     g.L = geo.L();
