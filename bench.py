@@ -171,7 +171,7 @@ def run_batch(args):
         wall = (time.perf_counter() - t0) / args.steps
         st = b.stats()
         log(f"batch of {len(idx)}: {wall * 1000:.1f} ms per run (kernel {kms / args.steps:.1f} ms, first run "
-            f"{cold:.2f} s, planning {st['plan_ms']:.0f} ms), {st['steps']:.0f} steps, {st['tiles']:.0f} tiles")
+            f"{cold:.2f} s, planning {st['plan_ms']:.0f} ms), {st['steps']:.0f} steps, {st['items']:.0f} queue items")
         return b, wall, kms / args.steps, cold
 
     os.environ["NCSV_JIT_MIN_QUBITS"] = "99"  # per-circuit runs: no per-circuit kernel compiles

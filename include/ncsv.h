@@ -296,9 +296,9 @@ ncsv_status ncsv_batch_run(ncsv_batch *b, const double *uniforms, const uint64_t
  * outcomes[d] = the outcome taken at draw d (0xff for noise draws). */
 ncsv_status ncsv_batch_result(const ncsv_batch *b, uint32_t entry, double *expect, double *norm_sq,
                               uint8_t *outcomes);
-/* out[8]: kernel ms, planning ms, tiles processed, launches, amplitude
+/* out[9]: kernel ms, planning ms, queue items processed, launches, amplitude
  * updates (gates x 2^n summed), HBM bytes of the multi-tile states, programs,
- * program steps. */
+ * program steps, launches replayed from a captured graph. */
 ncsv_status ncsv_batch_stats(const ncsv_batch *b, double *out);
 /* Plans every entry not planned yet (runs do this themselves; no device). */
 ncsv_status ncsv_batch_plan(ncsv_batch *b);

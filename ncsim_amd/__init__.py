@@ -561,7 +561,8 @@ class HipBatch:
         return [complex(out[2 * j], out[2 * j + 1]) for j in range(k)], nn.value, outc
 
     def stats(self):
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 9)()
         _check(L.lib().ncsv_batch_stats(self._h, out))
-        keys = ("kernel_ms", "plan_ms", "tiles", "launches", "amp_updates", "hbm_bytes", "programs", "steps")
+        keys = ("kernel_ms", "plan_ms", "items", "launches", "amp_updates", "hbm_bytes", "programs", "steps",
+                "graph_replays")
         return dict(zip(keys, list(out)))

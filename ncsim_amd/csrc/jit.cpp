@@ -773,7 +773,13 @@ bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t
         std::vector<uint32_t> idx;
     };
     std::vector<G0> g0;
-    for (uint32_t i = 0; i < count; i++) {
+    // first fit, widest X parts first (fewer tiles, so fewer passes over the state)
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return __builtin_popcountll(x[a] & ~low) > __builtin_popcountll(x[b] & ~low);
+    });
+    for (uint32_t i : order) {
         const uint64_t need = x[i] & ~low;
         bool placed = false;
         for (auto &g : g0)

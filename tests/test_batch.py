@@ -330,6 +330,28 @@ def test_gpu_batch_uneven_sizes_and_many_entries():
 
 
 @pytest.mark.gpu
+def test_gpu_graph_replays_match_eager(monkeypatch):
+    """Repeated runs replay the captured launch; every replay, with other
+    draws in between, equals an eager launch of the same draws."""
+    texts = [random_circuit(n, d, s) for n, d, s in CASES[:4]] + [random_circuit(15, 6, 77)]
+    paulis = [random_paulis(n, 3, s) for n, d, s in CASES[:4]] + [random_paulis(15, 3, 77)]
+    b = _build(texts, paulis, nc.C128, 0)
+    u1, u2 = _draws(b, 1), _draws(b, 2)
+    res = {}
+    for tag, us in (("a", u1), ("b", u2), ("a2", u1), ("b2", u2)):
+        b.run(uniforms=us)
+        res[tag] = [b.result(i) for i in range(len(b))]
+    assert b.stats()["graph_replays"] >= 3
+    monkeypatch.setenv("NCSV_BATCH_NO_GRAPH", "1")
+    for tag, us in (("a", u1), ("b", u2)):
+        b.run(uniforms=us)
+        eager = [b.result(i) for i in range(len(b))]
+        for got in (res[tag], res[tag + "2"]):
+            for (e1, n1, o1), (e2, n2, o2) in zip(got, eager):
+                assert o1 == o2 and n1 == n2 and e1 == e2
+
+
+@pytest.mark.gpu
 def test_gpu_autotune_keeps_results(tmp_path):
     """The autotuner times its candidates on a sample, applies one, caches
     it; results stay those of the oracle."""
