@@ -235,11 +235,196 @@ bool as_2x2(const StepH &s, const BOpD &o, uint32_t *pos, cd m[4]) {
     }
 }
 
+// The tile positions an op acts on, when it can join a dense block (no
+// condition on outer qubits, at most three positions); 0 otherwise.
+uint32_t block_support(const BOpD &o) {
+    uint32_t m = 0;
+    switch (o.kind) {
+    case BK_U1:
+    case BK_U1R:
+    case BK_U1X: m = 1u << o.p0; break;
+    case BK_U2:
+    case BK_SWAP: m = (1u << o.p0) | (1u << o.p1); break;
+    case BK_DM: m = o.omask ? 0u : o.lmask; break;
+    case BK_PERM1: m = o.omask ? 0u : (1u << o.p0) | o.lmask; break;
+    default: return 0;
+    }
+    return __builtin_popcount(m) <= 3 ? m : 0u;
+}
+
+// Applies one op to the 2^np amplitudes of positions pos[0..np) (local bit i
+// <-> position pos[i]), following the kernel's meaning of each op kind.
+void apply_local(const StepH &s, const BOpD &o, const uint32_t *pos, uint32_t np, cd *v) {
+    static const cd ip[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
+    auto lb = [&](uint32_t p) -> uint32_t {
+        for (uint32_t i = 0; i < np; i++)
+            if (pos[i] == p) return 1u << i;
+        fail(NCSV_INVALID, "batched planner: op outside its block");
+        return 0;
+    };
+    auto lmask = [&](uint32_t m) {
+        uint32_t r = 0;
+        for (; m; m &= m - 1) r |= lb((uint32_t)__builtin_ctz(m));
+        return r;
+    };
+    const uint32_t NA = 1u << np;
+    switch (o.kind) {
+    case BK_U1:
+    case BK_U1R:
+    case BK_U1X: {
+        const uint32_t b = lb(o.p0);
+        const cd *m = &s.coef[o.coef];
+        for (uint32_t x = 0; x < NA; x++) {
+            if (x & b) continue;
+            const cd a = v[x], c = v[x | b];
+            v[x] = m[0] * a + m[1] * c;
+            v[x | b] = m[2] * a + m[3] * c;
+        }
+        break;
+    }
+    case BK_U2: {
+        const uint32_t b0 = lb(o.p0), b1 = lb(o.p1);
+        const cd *m = &s.coef[o.coef];
+        for (uint32_t x = 0; x < NA; x++) {
+            if (x & (b0 | b1)) continue;
+            const uint32_t xs[4] = {x, x | b0, x | b1, x | b0 | b1};
+            cd w[4];
+            for (int r = 0; r < 4; r++) w[r] = v[xs[r]];
+            for (int r = 0; r < 4; r++) {
+                cd acc = 0;
+                for (int c = 0; c < 4; c++) acc += m[4 * r + c] * w[c];
+                v[xs[r]] = acc;
+            }
+        }
+        break;
+    }
+    case BK_DM: {
+        const uint32_t lm = lmask(o.lmask);
+        for (uint32_t x = 0; x < NA; x++)
+            if ((x & lm) == lm) v[x] *= s.coef[o.coef];
+        break;
+    }
+    case BK_PERM1: {
+        const uint32_t b = lb(o.p0), lm = lmask(o.lmask);
+        for (uint32_t x = 0; x < NA; x++) {
+            if ((x & b) || (x & lm) != lm) continue;
+            const cd a = v[x], c = v[x | b];
+            v[x | b] = ip[o.k0 & 3] * a;
+            v[x] = ip[o.k1 & 3] * c;
+        }
+        break;
+    }
+    case BK_SWAP: {
+        const uint32_t b0 = lb(o.p0), b1 = lb(o.p1);
+        for (uint32_t x = 0; x < NA; x++)
+            if ((x & b0) && !(x & b1)) std::swap(v[x], v[x ^ b0 ^ b1]);
+        break;
+    }
+    default: fail(NCSV_INVALID, "batched planner: op cannot join a block");
+    }
+}
+
+bool block_fusion_enabled() {
+    const char *e = getenv("NCSV_BATCH_MFMA");
+    return !(e && e[0] == '0');
+}
+
+// Complex64 steps: ops fused into dense 8x8 blocks on three tile positions
+// (BK_U3, FP32 MFMA in the kernel).  Each position belongs to at most one open
+// block; an op whose positions fit one block (together with the blocks it
+// touches) joins it, anything else closes the blocks it touches first, so a
+// block is emitted at its close with every op it commutes past acting on other
+// positions.  A closed block becomes one BK_U3 when it has two or more
+// positions, three or more ops and one op on two or more positions (a run of
+// one-position ops is cheaper as grouped 2x2s); otherwise its ops stay as
+// they were.
+std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
+    struct Blk {
+        uint32_t mask;
+        std::vector<BOpD> ops;
+    };
+    std::vector<BOpD> out;
+    std::vector<Blk> open;
+    auto emit = [&](const Blk &b) {
+        bool multi = false;
+        for (const BOpD &o : b.ops) multi |= __builtin_popcount(block_support(o)) >= 2;
+        const uint32_t np = (uint32_t)__builtin_popcount(b.mask);
+        if (np < 2 || b.ops.size() < 3 || !multi || s->L < 3) {
+            out.insert(out.end(), b.ops.begin(), b.ops.end());
+            return;
+        }
+        uint32_t pos[3], k = 0;
+        for (uint32_t m = b.mask; m; m &= m - 1) pos[k++] = (uint32_t)__builtin_ctz(m);
+        for (uint32_t p = 0; k < 3; p++)
+            if (!((b.mask >> p) & 1)) pos[k++] = p;  // identity on an unused position
+        BOpD d{};
+        d.kind = BK_U3;
+        d.p0 = (uint8_t)pos[0];
+        d.p1 = (uint8_t)pos[1];
+        d.p2 = (uint8_t)pos[2];
+        d.coef = (uint32_t)s->coef.size();
+        cd M[64];
+        for (uint32_t c = 0; c < 8; c++) {
+            cd v[8] = {};
+            v[c] = 1;
+            for (const BOpD &o : b.ops) apply_local(*s, o, pos, 3, v);
+            for (uint32_t r = 0; r < 8; r++) M[8 * r + c] = v[r];
+        }
+        for (int t = 0; t < 64; t++) s->coef.push_back(M[t]);
+        out.push_back(d);
+    };
+    auto close_if = [&](uint32_t m) {
+        for (size_t i = 0; i < open.size();) {
+            if (open[i].mask & m) {
+                emit(open[i]);
+                open.erase(open.begin() + (long)i);
+            } else {
+                i++;
+            }
+        }
+    };
+    for (const BOpD &o : in) {
+        const uint32_t sup = block_support(o);
+        if (!sup) {
+            // outer-conditioned ops and slots act on their own positions; the
+            // rest (measurement groups, expectations) on the whole tile
+            uint32_t m = ~0u;
+            if (o.kind == BK_NSLOT) m = 1u << o.p0;
+            else if (o.kind <= BK_SWAP) m = (1u << o.p0) | (1u << o.p1) | o.lmask;
+            if (o.kind == BK_DM) m = o.lmask;
+            close_if(m);
+            out.push_back(o);
+            continue;
+        }
+        uint32_t u = sup;
+        for (const Blk &b : open)
+            if (b.mask & sup) u |= b.mask;
+        if (__builtin_popcount(u) <= 3) {
+            Blk nb{u, {}};
+            for (size_t i = 0; i < open.size();) {
+                if (open[i].mask & sup) {
+                    nb.ops.insert(nb.ops.end(), open[i].ops.begin(), open[i].ops.end());
+                    open.erase(open.begin() + (long)i);
+                } else {
+                    i++;
+                }
+            }
+            nb.ops.push_back(o);
+            open.push_back(std::move(nb));
+        } else {
+            close_if(sup);
+            open.push_back(Blk{sup, {o}});
+        }
+    }
+    for (const Blk &b : open) emit(b);
+    return out;
+}
+
 // Runs of unconditional one-position ops become one pass over up to three
 // distinct positions (a later op on a position already in the run is
 // multiplied into its 2x2; the run's positions commute).  Noise slots are
 // kept, or dropped for the variant run by entries with no error here.
-std::vector<BOpD> group_ops(StepH *s, bool drop_slots) {
+std::vector<BOpD> group_ops(StepH *s, const std::vector<BOpD> &in, bool drop_slots) {
     std::vector<BOpD> out;
     std::vector<uint32_t> pos;
     std::vector<std::array<cd, 4>> mats;
@@ -263,7 +448,7 @@ std::vector<BOpD> group_ops(StepH *s, bool drop_slots) {
         mats.clear();
         orig.clear();
     };
-    for (const BOpD &o : s->ops) {
+    for (const BOpD &o : in) {
         if (drop_slots && o.kind == BK_NSLOT) continue;
         uint32_t p;
         cd m[4];
@@ -578,20 +763,29 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         for (size_t i = 0; i + 1 < P->steps.size(); i++)
             if (P->steps[i].modifies) P->steps[i].flags |= BF_STORE;
     }
+    const bool fuse = prec == NCSV_C64 && block_fusion_enabled();
     for (StepH &st : P->steps) {
-        st.gops = group_ops(&st, false);
-        st.gopsc = group_ops(&st, true);
+        if (fuse) {
+            std::vector<BOpD> clean;
+            for (const BOpD &o : st.ops)
+                if (o.kind != BK_NSLOT) clean.push_back(o);
+            st.gops = group_ops(&st, fuse_blocks(&st, st.ops), false);
+            st.gopsc = group_ops(&st, fuse_blocks(&st, clean), true);
+        } else {
+            st.gops = group_ops(&st, st.ops, false);
+            st.gopsc = group_ops(&st, st.ops, true);
+        }
     }
     if (getenv("NCSV_BATCH_DUMP")) {
         static const char *nm[] = {"U1", "U1R", "U1X", "U2", "DM", "PERM1", "SWAP", "NSLOT",
-                                   "MEAS", "RED", "COLL", "EXP", "G3"};
+                                   "MEAS", "RED", "COLL", "EXP", "G3", "U3"};
         for (size_t i = 0; i < P->steps.size(); i++) {
             const StepH &st = P->steps[i];
-            int cnt[13] = {0};
+            int cnt[14] = {0};
             for (const BOpD &o : st.gopsc) cnt[o.kind]++;
             fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu:", i, st.L, st.flags, st.gops.size(),
                     st.gopsc.size(), st.slots.size());
-            for (int k = 0; k < 13; k++)
+            for (int k = 0; k < 14; k++)
                 if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
             fprintf(stderr, "\n");
         }
@@ -766,7 +960,7 @@ Tables flatten(ncsv_batch *b) {
                            g0 = (uint32_t)t.mgroups.size();
             auto put = [&](const std::vector<BOpD> &ops) {
                 for (BOpD o : ops) {
-                    if (o.kind <= BK_DM || o.kind == BK_G3) o.coef += c0;
+                    if (o.kind <= BK_DM || o.kind == BK_G3 || o.kind == BK_U3) o.coef += c0;
                     if (o.kind == BK_EXPECT) o.c += p0;
                     if (o.kind == BK_MEASURE || o.kind == BK_REDUCE || o.kind == BK_COLLAPSE) o.c += g0;
                     t.ops.push_back(o);
@@ -1158,6 +1352,24 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                                 tl[x0] = C(4 * i) * a + C(4 * i + 1) * v;
                                 tl[x1] = C(4 * i + 2) * a + C(4 * i + 3) * v;
                             }
+                        break;
+                    }
+                    case BK_U3: {
+                        const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
+                        for (uint32_t x = 0; x < NL; x++) {
+                            if (x & (pb[0] | pb[1] | pb[2])) continue;
+                            uint32_t xs[8];
+                            cd v[8];
+                            for (uint32_t a = 0; a < 8; a++) {
+                                xs[a] = x | ((a & 1) ? pb[0] : 0) | ((a & 2) ? pb[1] : 0) | ((a & 4) ? pb[2] : 0);
+                                v[a] = tl[xs[a]];
+                            }
+                            for (uint32_t r = 0; r < 8; r++) {
+                                cd acc = 0;
+                                for (uint32_t c = 0; c < 8; c++) acc += C(8 * r + c) * v[c];
+                                tl[xs[r]] = acc;
+                            }
+                        }
                         break;
                     }
                     case BK_U1:

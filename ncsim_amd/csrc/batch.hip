@@ -11,6 +11,7 @@ namespace kern {
 template <typename T> struct V2;
 template <> struct V2<float> { using type = float2; };
 template <> struct V2<double> { using type = double2; };
+typedef float F4 __attribute__((ext_vector_type(4)));
 }  // namespace kern
 }  // namespace batch
 }  // namespace ncsv
@@ -32,6 +33,7 @@ template <> struct V2<double> { using type = double2; };
 #define NCSV_UNIFORM32(v) __builtin_amdgcn_readfirstlane((uint32_t)(v))
 #define NCSV_LANE0_STORE(dst, v) ((dst) = ::ncsv::batch::kern::uniform64(v))
 #define NCSV_OPAQUE(x) asm volatile("" : "+v"(x))
+#define NCSV_MFMA16(a, b, acc) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (acc), 0, 0, 0)
 
 namespace ncsv {
 namespace batch {

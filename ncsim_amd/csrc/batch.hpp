@@ -43,6 +43,7 @@ enum BOpKind : uint8_t {
     BK_COLLAPSE = 10,  // group c decided from b tiles of cell weights at a; collapse, resets
     BK_EXPECT = 11, // Paulis [c, c+b) to part[a + 2b tile + 2k]
     BK_G3 = 12,     // k1 (2 or 3) unconditional 2x2s on distinct positions p0, p1, p2 in one pass
+    BK_U3 = 13,     // dense 8x8 on positions p0 < p1 < p2 (coef: 64, row-major; complex64 on FP32 MFMA)
 };
 
 struct BOpD {

@@ -9,7 +9,9 @@
 //   NCSV_FENCE_RELEASE(), NCSV_SLEEP(), NCSV_UNIFORM32(v) / NCSV_UNIFORM64(v)
 //   (the value of the wavefront's first lane), NCSV_LANE0_STORE(dst, v)
 //   (dst = lane 0's v), NCSV_OPAQUE(x) (x unknown to the optimizer from
-//   here on), V2<T>::type (float2 / double2).
+//   here on), NCSV_MFMA16(a, b, acc) (one v_mfma_f32_16x16x4_f32: lane l
+//   holds A[l % 16][l / 16] and B[l / 16][l % 16]; acc[v] is D[4 (l / 16) + v][l % 16]),
+//   V2<T>::type (float2 / double2), F4 (4 floats).
 //
 // The tile sits in LDS at swizzled slots: slot(x) = x ^ f(x >> S), an
 // XOR-linear bijection that spreads the strided patterns of pair, quad and
@@ -359,6 +361,56 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                         visit<V>(NL, tid, pb[0] | pb[1] | pb[2], 0, [&](uint32_t, uint32_t s) { body(s, KTag<3>{}); });
                     else
                         visit<V>(NL, tid, pb[0] | pb[1], 0, [&](uint32_t, uint32_t s) { body(s, KTag<2>{}); });
+                    break;
+                }
+                case BK_U3: {
+                    // dense 8x8 on positions p0, p1, p2 (local bit i <-> p_i), complex64 only:
+                    // per 16 groups of 8 amplitudes, D (16 x 16) = A (16 x 16) x B (16 x 16) in
+                    // four FP32 MFMAs, A the real form of the 8x8.  Row or column 4q + c of a
+                    // group <-> amplitude 2q + (c >> 1), real part for even c, imaginary for odd
+                    // c, so lane l (group l % 16, q = l / 16) reads and writes amplitudes 2q and
+                    // 2q + 1 of its group and nothing else.
+                    if constexpr (sizeof(T) == 4) {
+                        const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
+                        const uint32_t mask = pb[0] | pb[1] | pb[2];
+                        const uint32_t lane = tid & 63, q = lane >> 4, wv = tid >> 6;
+                        float av[4];
+#pragma unroll
+                        for (uint32_t kb = 0; kb < 4; kb++) {
+                            // A[lane % 16][4 kb + q]
+                            const uint32_t i = lane & 15, ai = 2 * (i >> 2) + ((i >> 1) & 1), ci = i & 1;
+                            const uint32_t ak = 2 * q + (kb >> 1), ck = kb & 1;
+                            const uint32_t e = op.coef + 8 * ai + ak;
+                            const double re = cf[2 * e], im = cf[2 * e + 1];
+                            av[kb] = (float)(ci == ck ? re : (ci ? im : -im));
+                        }
+                        auto off = [&](uint32_t a) {
+                            return sw<V>(((a & 1) ? pb[0] : 0u) | ((a & 2) ? pb[1] : 0u) | ((a & 4) ? pb[2] : 0u));
+                        };
+                        const uint32_t sa0 = off(2 * q), sa1 = off(2 * q + 1);
+                        const uint32_t ngroups = NL >> 3, gl = (lane & 15) + 16 * wv;
+                        const uint32_t nrounds = (ngroups + 16 * NW - 1) / (16 * NW);  // the same in every wavefront
+                        const bool valid = gl < ngroups;  // every group is valid when there are two rounds or more
+                        const uint32_t st = sw<V>(deposit(valid ? gl : 0u, mask, 0));
+                        for (uint32_t rd = 0; rd < nrounds; rd++) {
+                            const uint32_t sh = st ^ sw<V>(deposit(rd * 16 * NW, mask, 0));
+                            const V in0 = tile[sh ^ sa0], in1 = tile[sh ^ sa1];
+                            F4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                            acc = NCSV_MFMA16(av[0], valid ? in0.x : 0.0f, acc);
+                            acc = NCSV_MFMA16(av[1], valid ? in0.y : 0.0f, acc);
+                            acc = NCSV_MFMA16(av[2], valid ? in1.x : 0.0f, acc);
+                            acc = NCSV_MFMA16(av[3], valid ? in1.y : 0.0f, acc);
+                            if (valid) {
+                                V o0, o1;
+                                o0.x = acc[0];
+                                o0.y = acc[1];
+                                o1.x = acc[2];
+                                o1.y = acc[3];
+                                tile[sh ^ sa0] = o0;
+                                tile[sh ^ sa1] = o1;
+                            }
+                        }
+                    }  // complex128 programs never hold this op
                     break;
                 }
                 case BK_DM: {
