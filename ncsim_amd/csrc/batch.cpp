@@ -371,6 +371,17 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
             for (uint32_t r = 0; r < 8; r++) M[8 * r + c] = v[r];
         }
         for (int t = 0; t < 64; t++) s->coef.push_back(M[t]);
+        // the kernel's FP32 copy (four floats per slot: row r, columns 2q and 2q + 1)
+        d.a = (uint32_t)s->coef.size();
+        for (uint32_t r = 0; r < 8; r++)
+            for (uint32_t q = 0; q < 4; q++) {
+                const float f[4] = {(float)M[8 * r + 2 * q].real(), (float)M[8 * r + 2 * q].imag(),
+                                    (float)M[8 * r + 2 * q + 1].real(), (float)M[8 * r + 2 * q + 1].imag()};
+                cd slot;
+                static_assert(sizeof(slot) == sizeof(f), "four floats per complex slot");
+                memcpy((void *)&slot, f, sizeof(f));
+                s->coef.push_back(slot);
+            }
         out.push_back(d);
     };
     auto close_if = [&](uint32_t m) {
@@ -956,11 +967,16 @@ Tables flatten(ncsv_batch *b) {
             d.n = P->n;
             d.flags = s.flags;
             d.qoff = (uint32_t)t.qtab.size();
+            for (uint32_t i = 0; i < s.qt.size(); i++) {
+                if (i < s.L && i < 16) d.tpos[i] = s.qt[i];
+                if (i >= s.L) d.omask |= 1u << s.qt[i];
+            }
             const uint32_t c0 = (uint32_t)(t.coef.size() / 2), p0 = (uint32_t)t.paulis.size(),
                            g0 = (uint32_t)t.mgroups.size();
             auto put = [&](const std::vector<BOpD> &ops) {
                 for (BOpD o : ops) {
                     if (o.kind <= BK_DM || o.kind == BK_G3 || o.kind == BK_U3) o.coef += c0;
+                    if (o.kind == BK_U3) o.a += c0;
                     if (o.kind == BK_EXPECT) o.c += p0;
                     if (o.kind == BK_MEASURE || o.kind == BK_REDUCE || o.kind == BK_COLLAPSE) o.c += g0;
                     t.ops.push_back(o);
@@ -1158,7 +1174,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
         return o;
     };
     const size_t o_ent = take(ch.ed.size() * sizeof(BEntryD)), o_rec = take(ch.recs.size() * sizeof(BRecD)),
-                 o_blk = take(ch.rec_blk.size() * 4), o_part = take((ch.npart + 1) * 8),
+                 o_blk = take(ch.rec_blk.size() * 4), o_part = take((ch.npart + 16) * 8),
                  o_exp = take((ch.nexp + 1) * 8), o_lsc = take(ch.lsc.size() * 8), o_uni = take(ch.uni.size() * 8),
                  o_forced = take(ch.forced.size()), o_outc = take(ch.nmeas + 1), o_errs = take(ch.errs.size()),
                  o_done = take(ch.recs.size() * 4), o_next = take(8), o_flag = take(4);
@@ -1254,7 +1270,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
 void run_chunk_sim(ncsv_batch *b, const Tables &t, Chunk &ch) {
     const size_t es = b->prec == NCSV_C128 ? 16 : 8;
     std::vector<double> amp(std::max<size_t>(2, ch.amp_elems * es / 8), 0.0);
-    std::vector<double> part(ch.npart + 1, 0.0);
+    std::vector<double> part(ch.npart + 16, 0.0);
     ch.pexp.assign(ch.nexp + 1, 0.0);
     ch.outc.assign(ch.nmeas + 1, 0xff);
     std::vector<uint32_t> done(ch.recs.size(), 0);
@@ -1310,7 +1326,7 @@ uint32_t parity(uint64_t v) { return (uint32_t)__builtin_popcountll(v) & 1u; }
 void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
     ch.pexp.assign(ch.nexp + 1, 0.0);
     ch.outc.assign(ch.nmeas + 1, 0xff);
-    std::vector<double> part(ch.npart + 1, 0.0);
+    std::vector<double> part(ch.npart + 16, 0.0);
     const bool c64 = b->prec == NCSV_C64;
     for (size_t i = 0; i < ch.ent.size(); i++) {
         const Program &P = *b->progs[b->entries[ch.ent[i]].prog];
@@ -1651,7 +1667,7 @@ ncsv_status ncsv_batch_add(ncsv_batch *b, const ncsv_circuit *c, uint32_t count,
     return ncsv::guard_status([&]() {
         if (!b || !c) ncsv::fail(NCSV_INVALID, "null batch or circuit");
         if (count && (!x || !z || !phase)) ncsv::fail(NCSV_INVALID, "null Pauli arrays");
-        if (c->n == 0 || c->n > 34) ncsv::fail(NCSV_INVALID, "batched form: 1..34 qubits per circuit");
+        if (c->n == 0 || c->n > 32) ncsv::fail(NCSV_INVALID, "batched form: 1..32 qubits per circuit");
         const uint64_t full = c->n >= 64 ? ~0ull : (1ull << c->n) - 1;
         for (uint32_t k = 0; k < count; k++)
             if ((x[k] | z[k]) & ~full) ncsv::fail(NCSV_INVALID, "Pauli acts outside the circuit's qubits");

@@ -84,10 +84,10 @@ void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipS
     check_hip(hipGetLastError(), "k_batch_reset");
     if (prec == NCSV_C128)
         hipLaunchKernelGGL((k_batch<double, 12>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
-                           A.qtab, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
+                           A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
     else
         hipLaunchKernelGGL((k_batch<float, 13>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
-                           A.qtab, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
+                           A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
     check_hip(hipGetLastError(), "k_batch");
 }
 

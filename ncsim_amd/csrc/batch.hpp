@@ -82,7 +82,10 @@ struct BStepD {
     uint32_t op0, nops;    // the ops
     uint32_t op0c, nopsc;  // the same without noise slots (entries with no error drawn in this step)
     uint32_t flags;
-    uint32_t qoff;  // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
+    uint32_t qoff;   // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
+    uint32_t omask;  // the outer qubits (tile number bit b <-> the b-th lowest)
+    uint32_t pad;
+    uint8_t tpos[16];  // the qubits of tile positions 0..min(L, 16)-1
 };
 
 struct BEntryD {

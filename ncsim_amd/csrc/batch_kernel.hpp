@@ -97,10 +97,14 @@ NCSV_DEV inline void visit(uint32_t NL, uint32_t tid, uint32_t mask, uint32_t on
         return;
     }
     const uint32_t st = sw<V>(xt);
+    // h = deposit(i * BT, mask, 0) by a masked increment: the positions of
+    // mask and of the thread's bits are held at one while adding
+    const uint32_t fill = mask | deposit(BT - 1, mask, 0);
+    uint32_t h = 0;
 #pragma unroll 2
     for (uint32_t i = 0; i < cnt / BT; i++) {
-        const uint32_t h = deposit(i * BT, mask, 0);
         f(xt | h, st ^ sw<V>(h));
+        h = ((h | fill) + 1) & ~fill;
     }
 }
 
@@ -146,7 +150,7 @@ NCSV_DEV inline uint32_t decide(const double *__restrict__ uni, const uint8_t *_
 template <typename T, uint32_t LB>
 NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restrict__ ops,
                          const double *__restrict__ cf, const BPauliD *__restrict__ paulis,
-                         const BMGroupD *__restrict__ mgroups, const uint8_t *__restrict__ qtab, const BEntryD *__restrict__ entries,
+                         const BMGroupD *__restrict__ mgroups, const BEntryD *__restrict__ entries,
                          const BRecD *__restrict__ recs, const uint32_t *__restrict__ rec_blk,
                          const double *__restrict__ uni, const uint8_t *__restrict__ forced,
                          const uint8_t *__restrict__ errs, BArgs A) {
@@ -198,58 +202,73 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
         const BStepD st = steps[rec.step];
         const BEntryD en = entries[rec.entry];
         const uint32_t L = st.L, NL = 1u << L;
-        const uint8_t *qt = qtab + st.qoff;
         V *g = (V *)A.amp + en.amp_off;
-        uint64_t offlo = 0;
-        for (uint32_t p = 0; p < L && p < LOB; p++)
-            if ((tid >> p) & 1) offlo |= 1ull << qt[p];
-        // memory strides of the tile positions above the thread bits
-        uint64_t wq[HB];
+        // element offsets within a state fit 32 bits (at most 32 qubits): the
+        // thread's own bits, and the wave-uniform strides of the tile
+        // positions above the thread bits
+        uint32_t offlo = 0;
 #pragma unroll
-        for (uint32_t b = 0; b < HB; b++) wq[b] = (b + LOB < L) ? 1ull << qt[LOB + b] : 0ull;
-        const uint32_t kcnt = L >= LOB ? 1u << (L - LOB) : (tid < NL ? 1u : 0u);
+        for (uint32_t p = 0; p < LOB; p++)
+            if (p < L && ((tid >> p) & 1)) offlo |= 1u << st.tpos[p];
+        uint32_t wq[HB];
+#pragma unroll
+        for (uint32_t b = 0; b < HB; b++) wq[b] = (b + LOB < L) ? 1u << st.tpos[LOB + b] : 0u;
+        const bool full = L >= LOB;  // every thread holds 2^(L - LOB) amplitudes
+        const uint32_t kcnt = full ? 1u << (L - LOB) : 0u;
+        constexpr uint32_t CH = sizeof(T) == 4 ? 8u : 4u;
+        // the memory offset of the high tile bits k
+        auto hibits = [&](uint32_t k) {
+            uint32_t h = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < HB; b++)
+                if ((k >> b) & 1) h |= wq[b];
+            return h;
+        };
         const uint32_t op0 = (rec.flags & RF_ERR) ? st.op0 : st.op0c;
         const uint32_t op1 = op0 + ((rec.flags & RF_ERR) ? st.nops : st.nopsc);
         // each thread reads back only the LDS slots it wrote, so consecutive
         // tiles need no barrier between the store of one and the load of the next
         for (uint32_t ti = 0; ti < ntl; ti++) {
             const uint64_t tileno = t0 + ti;
-            uint64_t base = 0;
-            for (uint32_t b = 0; b + L < st.n; b++)
-                if ((tileno >> b) & 1) base |= 1ull << qt[L + b];
+            // the outer qubits' offset: tile number bits deposited on omask
+            uint32_t base = 0;
+            {
+                uint64_t tn = tileno;
+                for (uint32_t m = st.omask; m; m &= m - 1, tn >>= 1)
+                    if (tn & 1) base |= m & (0u - m);
+            }
 
+            // the tile's first amplitude; the thread's offset recomputed per tile
+            // so that no per-load address stays live across the ops
+            const V *gt = g + base;
+            uint32_t lo = offlo;
+            NCSV_OPAQUE(lo);
             if (st.flags & BF_INIT) {
-#pragma unroll
-                for (uint32_t k = 0; k < KH; k++) {
-                    if (k < kcnt) {
-                        V v;
-                        v.x = (k == 0 && tid == 0 && tileno == 0) ? (T)1 : (T)0;
-                        v.y = (T)0;
-                        tile[st_tid ^ sw<V>(BT * k)] = v;
-                    }
+                V z;
+                z.x = (T)0;
+                z.y = (T)0;
+                if (full) {
+                    for (uint32_t k = 0; k < kcnt; k++) tile[st_tid ^ sw<V>(BT * k)] = z;
+                } else if (tid < NL) {
+                    tile[st_tid] = z;
                 }
-            } else {
-                // loads issued in chunks before any of them is waited on
-                constexpr uint32_t CH = sizeof(T) == 4 ? 8u : 4u;
-                const uint64_t tb = base + offlo;
-#pragma unroll
-                for (uint32_t k0 = 0; k0 < KH; k0 += CH) {
+                if (tid == 0 && tileno == 0) tile[0].x = (T)1;
+            } else if (kcnt >= CH) {
+                // loads issued in chunks of CH before any of them is waited
+                // on; the chunk loop and every address term but the thread's
+                // own are wave-uniform
+                for (uint32_t k0 = 0; k0 < kcnt; k0 += CH) {
+                    const uint32_t h0 = lo | hibits(k0);
                     V v[CH];
 #pragma unroll
-                    for (uint32_t c = 0; c < CH; c++) {
-                        const uint32_t k = k0 + c;
-                        if (k < kcnt) {
-                            uint64_t off = tb;
+                    for (uint32_t c = 0; c < CH; c++) v[c] = gt[h0 | hibits(c)];
 #pragma unroll
-                            for (uint32_t b = 0; b < HB; b++)
-                                if ((k >> b) & 1) off += wq[b];
-                            v[c] = g[off];
-                        }
-                    }
-#pragma unroll
-                    for (uint32_t c = 0; c < CH; c++)
-                        if (k0 + c < kcnt) tile[st_tid ^ sw<V>(BT * (k0 + c))] = v[c];
+                    for (uint32_t c = 0; c < CH; c++) tile[st_tid ^ sw<V>(BT * (k0 + c))] = v[c];
                 }
+            } else if (full) {
+                for (uint32_t k = 0; k < kcnt; k++) tile[st_tid ^ sw<V>(BT * k)] = gt[lo | hibits(k)];
+            } else if (tid < NL) {
+                tile[st_tid] = gt[lo];
             }
             NCSV_SYNC();
 
@@ -374,26 +393,18 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                         const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
                         const uint32_t mask = pb[0] | pb[1] | pb[2];
                         const uint32_t lane = tid & 63, q = lane >> 4, wv = tid >> 6;
-                        float av[4];
-#pragma unroll
-                        for (uint32_t kb = 0; kb < 4; kb++) {
-                            // A[lane % 16][4 kb + q]
-                            const uint32_t i = lane & 15, ai = 2 * (i >> 2) + ((i >> 1) & 1), ci = i & 1;
-                            const uint32_t ak = 2 * q + (kb >> 1), ck = kb & 1;
-                            const uint32_t e = op.coef + 8 * ai + ak;
-                            const double re = cf[2 * e], im = cf[2 * e + 1];
-                            av[kb] = (float)(ci == ck ? re : (ci ? im : -im));
-                        }
+                        // A[lane % 16][4 kb + q] from the FP32 copy of the block (row-major
+                        // complex pairs at op.a): the lane's row ai, columns 2q and 2q + 1
+                        const uint32_t i = lane & 15, ai = 2 * (i >> 2) + ((i >> 1) & 1), ci = i & 1;
+                        const float *ur = (const float *)(cf + 2 * op.a) + 16 * ai + 4 * q;
+                        const float u0r = ur[0], u0i = ur[1], u1r = ur[2], u1i = ur[3];
+                        const float av[4] = {ci ? u0i : u0r, ci ? u0r : -u0i, ci ? u1i : u1r, ci ? u1r : -u1i};
                         auto off = [&](uint32_t a) {
                             return sw<V>(((a & 1) ? pb[0] : 0u) | ((a & 2) ? pb[1] : 0u) | ((a & 4) ? pb[2] : 0u));
                         };
                         const uint32_t sa0 = off(2 * q), sa1 = off(2 * q + 1);
                         const uint32_t ngroups = NL >> 3, gl = (lane & 15) + 16 * wv;
-                        const uint32_t nrounds = (ngroups + 16 * NW - 1) / (16 * NW);  // the same in every wavefront
-                        const bool valid = gl < ngroups;  // every group is valid when there are two rounds or more
-                        const uint32_t st = sw<V>(deposit(valid ? gl : 0u, mask, 0));
-                        for (uint32_t rd = 0; rd < nrounds; rd++) {
-                            const uint32_t sh = st ^ sw<V>(deposit(rd * 16 * NW, mask, 0));
+                        auto round = [&](uint32_t sh, bool valid) {
                             const V in0 = tile[sh ^ sa0], in1 = tile[sh ^ sa1];
                             F4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                             acc = NCSV_MFMA16(av[0], valid ? in0.x : 0.0f, acc);
@@ -409,6 +420,23 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                                 tile[sh ^ sa0] = o0;
                                 tile[sh ^ sa1] = o1;
                             }
+                        };
+                        if (ngroups >= 16 * NW) {
+                            // every group valid; round rd covers groups 16 NW rd + gl, its
+                            // uniform part stepped by a masked increment
+                            const uint32_t st = sw<V>(deposit(gl, mask, 0));
+                            const uint32_t fill = mask | deposit(16 * NW - 1, mask, 0);
+                            const uint32_t nrounds = ngroups / (16 * NW);
+                            uint32_t h = 0;
+#pragma unroll 2
+                            for (uint32_t rd = 0; rd < nrounds; rd++) {
+                                round(st ^ sw<V>(h), true);
+                                h = ((h | fill) + 1) & ~fill;
+                            }
+                        } else {
+                            // one round; the same MFMAs in every wavefront
+                            const bool valid = gl < ngroups;
+                            round(sw<V>(deposit(valid ? gl : 0u, mask, 0)), valid);
                         }
                     }  // complex128 programs never hold this op
                     break;
@@ -479,11 +507,15 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     for (uint32_t c = 0; c < CMAX; c++) v[c] = 0.0;
                     if (op.kind == BK_COLLAPSE) {
                         // the cell weights from every tile's partials, same order in each tile
+                        // 16 loads per tile without per-cell branches (the partials
+                        // are padded by 16 doubles; cells from NC on are dropped)
                         const double *pp = A.part + en.part0 + op.a;
                         for (uint32_t t = tid; t < op.b; t += BT) {
+                            double x[16];
 #pragma unroll
-                            for (uint32_t c = 0; c < 16; c++)
-                                if (c < NC) v[c] += pp[NC * t + c];
+                            for (uint32_t c = 0; c < 16; c++) x[c] = pp[NC * t + c];
+#pragma unroll
+                            for (uint32_t c = 0; c < 16; c++) v[c] += c < NC ? x[c] : 0.0;
                         }
                     } else {
                         visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
@@ -598,17 +630,19 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
             }
 
             if (st.flags & BF_STORE) {
-                uint64_t tb = base + offlo;
-                NCSV_OPAQUE(tb);  // recomputed here, not held live across the ops
+                V *gs = g + base;
+                uint32_t los = offlo;
+                NCSV_OPAQUE(los);  // recomputed here, not held live across the ops
+                if (kcnt >= CH) {
+                    for (uint32_t k0 = 0; k0 < kcnt; k0 += CH) {
+                        const uint32_t h0 = los | hibits(k0);
 #pragma unroll
-                for (uint32_t k = 0; k < KH; k++) {
-                    if (k < kcnt) {
-                        uint64_t off = tb;
-#pragma unroll
-                        for (uint32_t b = 0; b < HB; b++)
-                            if ((k >> b) & 1) off += wq[b];
-                        g[off] = tile[st_tid ^ sw<V>(BT * k)];
+                        for (uint32_t c = 0; c < CH; c++) gs[h0 | hibits(c)] = tile[st_tid ^ sw<V>(BT * (k0 + c))];
                     }
+                } else if (full) {
+                    for (uint32_t k = 0; k < kcnt; k++) gs[los | hibits(k)] = tile[st_tid ^ sw<V>(BT * k)];
+                } else if (tid < NL) {
+                    gs[los] = tile[st_tid];
                 }
             }
         }  // tiles of the item

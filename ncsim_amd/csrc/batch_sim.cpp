@@ -121,10 +121,10 @@ void simulate(const BArgs &A, ncsv_precision prec) {
         th.emplace_back([&A, prec, t]() {
             sim::tid = t;
             if (prec == NCSV_C128)
-                kern::k_batch<double, 12>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.qtab, A.entries, A.recs, A.rec_blk,
+                kern::k_batch<double, 12>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.entries, A.recs, A.rec_blk,
                                           A.uni, A.forced, A.errs, A);
             else
-                kern::k_batch<float, 13>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.qtab, A.entries, A.recs, A.rec_blk,
+                kern::k_batch<float, 13>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.entries, A.recs, A.rec_blk,
                                          A.uni, A.forced, A.errs, A);
         });
     }
