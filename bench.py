@@ -253,6 +253,14 @@ def run_traj(args):
         return b
 
     b64 = make(H.C64, T)
+    tuned = {}
+    if args.tile_bits:
+        b64.set_tile_bits(args.tile_bits)
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        r = b64.autotune(log=log, sample=48)
+        tuned["c64"] = {"config": r["config"], "cached": r["cached"], "seconds": time.perf_counter() - t0}
+        log(f"autotune complex64: {r['config']} ({'cached' if r['cached'] else 'measured'})")
     seeds = np.arange(T, dtype=np.uint64) + 1000
     b64.run(seeds=seeds)  # warm-up (planning, programs)
     t0 = time.perf_counter()
@@ -268,6 +276,10 @@ def run_traj(args):
         f"{st['launches']:.0f} launches)")
     m = min(256, T)
     b128 = make(H.C128, m)
+    if not args.tile_bits and not args.no_autotune:
+        r = b128.autotune(log=log, sample=48)
+        tuned["c128"] = {"config": r["config"], "cached": r["cached"]}
+        log(f"autotune complex128: {r['config']} ({'cached' if r['cached'] else 'measured'})")
     forced = []
     for i in range(m):
         oc = b64.result(i)[2]
@@ -281,6 +293,9 @@ def run_traj(args):
     # per-trajectory launches: the same circuit through the single-state run
     # (noise ops split its fused segments), its own SplitMix64 draws
     single = min(args.single, T)
+    # per-trajectory runs through the interpreter kernels: each trajectory's noise
+    # splits the circuit differently, so specialized kernels would compile per run
+    os.environ["NCSV_JIT_MIN_QUBITS"] = "99"
     s = H.HipStateVector.zero_state(n, H.C64)
     ms_single = None
     if single:
@@ -305,6 +320,7 @@ def run_traj(args):
            "estimates": {"mean_Z": est64[:, :n].mean(axis=0).tolist()[:4],
                          "mean_ZZ": est64[:, n:].mean(axis=0).tolist()[:4]},
            "fp64_check": {"trajectories": m, "max_abs_estimate_diff": diff, "seconds_c128": w128},
+           "autotune": tuned,
            "per_trajectory": {"ms_per_trajectory": ms_single, "trajectories": single,
                               "speedup_batched": (ms_single / (wall * 1000 / T)) if ms_single else None}}
     print(json.dumps(out), flush=True)

@@ -283,7 +283,7 @@ ncsv_status ncsv_batch_draws(const ncsv_batch *b, uint32_t entry, uint32_t *coun
 ncsv_status ncsv_batch_set_tile_bits(ncsv_batch *b, uint32_t bits);
 ncsv_status ncsv_batch_set_blocks(ncsv_batch *b, int32_t blocks);
 /* The tile's fixed low qubits (coalescing against fewer sweeps; -1 = 5 for
- * complex64, 4 for complex128) and the most tiles per queue item (-1 = 8). */
+ * complex64, 4 for complex128) and the most tiles per queue item (-1 = 32). */
 ncsv_status ncsv_batch_set_shape(ncsv_batch *b, int32_t low_bits, int32_t tiles_per_item);
 /* Runs every entry.  `uniforms` holds each entry's draws back to back (in
  * entry order), or is NULL and entry e's draws come from a SplitMix64 stream

@@ -53,7 +53,7 @@ def candidates(precision):
     out = []
     for bits in (top, top - 1):
         for lo in (low, 3):
-            for items in (8, 32):
+            for items in (32, 128):
                 out.append({"tile_bits": bits, "low_bits": lo, "tiles_per_item": items})
     return out
 

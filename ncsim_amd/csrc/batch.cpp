@@ -235,8 +235,8 @@ bool as_2x2(const StepH &s, const BOpD &o, uint32_t *pos, cd m[4]) {
     }
 }
 
-// The tile positions an op acts on, when it can join a dense block (no
-// condition on outer qubits, at most three positions); 0 otherwise.
+// The tile positions an op acts on, when it can join a dense block (at most
+// three positions; a condition on outer qubits is allowed); 0 otherwise.
 uint32_t block_support(const BOpD &o) {
     uint32_t m = 0;
     switch (o.kind) {
@@ -245,16 +245,18 @@ uint32_t block_support(const BOpD &o) {
     case BK_U1X: m = 1u << o.p0; break;
     case BK_U2:
     case BK_SWAP: m = (1u << o.p0) | (1u << o.p1); break;
-    case BK_DM: m = o.omask ? 0u : o.lmask; break;
-    case BK_PERM1: m = o.omask ? 0u : (1u << o.p0) | o.lmask; break;
+    case BK_DM: m = o.lmask; break;
+    case BK_PERM1: m = (1u << o.p0) | o.lmask; break;
     default: return 0;
     }
     return __builtin_popcount(m) <= 3 ? m : 0u;
 }
 
 // Applies one op to the 2^np amplitudes of positions pos[0..np) (local bit i
-// <-> position pos[i]), following the kernel's meaning of each op kind.
-void apply_local(const StepH &s, const BOpD &o, const uint32_t *pos, uint32_t np, cd *v) {
+// <-> position pos[i]), following the kernel's meaning of each op kind, in a
+// tile whose outer qubits meet the op's outer condition (outer_true) or not.
+void apply_local(const StepH &s, const BOpD &o, const uint32_t *pos, uint32_t np, cd *v, bool outer_true) {
+    if (o.omask && !outer_true) return;  // the tile's outer qubits fail the op's condition
     static const cd ip[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
     auto lb = [&](uint32_t p) -> uint32_t {
         for (uint32_t i = 0; i < np; i++)
@@ -329,8 +331,8 @@ bool block_fusion_enabled() {
     return !(e && e[0] == '0');
 }
 
-// Complex64 steps: ops fused into dense 8x8 blocks on three tile positions
-// (BK_U3, FP32 MFMA in the kernel).  Each position belongs to at most one open
+// Ops fused into dense 8x8 blocks on three tile positions (BK_U3: FP32 MFMA
+// in complex64 programs, FP64 MFMA in complex128 ones).  Each position belongs to at most one open
 // block; an op whose positions fit one block (together with the blocks it
 // touches) joins it, anything else closes the blocks it touches first, so a
 // block is emitted at its close with every op it commutes past acting on other
@@ -338,18 +340,46 @@ bool block_fusion_enabled() {
 // positions, three or more ops and one op on two or more positions (a run of
 // one-position ops is cheaper as grouped 2x2s); otherwise its ops stay as
 // they were.
-std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
+std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in, bool fp32_copy) {
     struct Blk {
         uint32_t mask;
         std::vector<BOpD> ops;
+        uint64_t omask = 0;  // the one outer condition of its ops (0: none)
     };
+    // blocks may join when their outer conditions agree or one has none
+    auto agree = [](uint64_t a, uint64_t b) { return !a || !b || a == b; };
     std::vector<BOpD> out;
     std::vector<Blk> open;
-    auto emit = [&](const Blk &b) {
+    // the last emitted block while nothing was emitted after it: its ops fill
+    // out[start..), as one dense block or as they were
+    struct Prev {
+        bool valid = false, dense = false;
+        size_t start = 0;
+        Blk b{0, {}};
+    } prev;
+    auto worthy = [&](const Blk &b) {
         bool multi = false;
         for (const BOpD &o : b.ops) multi |= __builtin_popcount(block_support(o)) >= 2;
-        const uint32_t np = (uint32_t)__builtin_popcount(b.mask);
-        if (np < 2 || b.ops.size() < 3 || !multi || s->L < 3) {
+        return __builtin_popcount(b.mask) >= 2 && b.ops.size() >= 3 && multi && s->L >= 3;
+    };
+    auto emit = [&](const Blk &b0) {
+        Blk b = b0;
+        bool dense = worthy(b);
+        if (prev.valid && !(prev.b.mask & b.mask) && __builtin_popcount(prev.b.mask | b.mask) <= 3 &&
+            agree(prev.b.omask, b.omask) && (prev.dense || dense)) {
+            // the previous block acts on other positions and nothing ran in
+            // between: one dense block for both (they commute)
+            b.ops.insert(b.ops.begin(), prev.b.ops.begin(), prev.b.ops.end());
+            b.mask |= prev.b.mask;
+            b.omask |= prev.b.omask;
+            out.resize(prev.start);
+            dense = true;
+        }
+        prev.valid = true;
+        prev.dense = dense;
+        prev.start = out.size();
+        prev.b = b;
+        if (!dense) {
             out.insert(out.end(), b.ops.begin(), b.ops.end());
             return;
         }
@@ -362,26 +392,33 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
         d.p0 = (uint8_t)pos[0];
         d.p1 = (uint8_t)pos[1];
         d.p2 = (uint8_t)pos[2];
-        d.coef = (uint32_t)s->coef.size();
-        cd M[64];
-        for (uint32_t c = 0; c < 8; c++) {
-            cd v[8] = {};
-            v[c] = 1;
-            for (const BOpD &o : b.ops) apply_local(*s, o, pos, 3, v);
-            for (uint32_t r = 0; r < 8; r++) M[8 * r + c] = v[r];
-        }
-        for (int t = 0; t < 64; t++) s->coef.push_back(M[t]);
-        // the kernel's FP32 copy (four floats per slot: row r, columns 2q and 2q + 1)
-        d.a = (uint32_t)s->coef.size();
-        for (uint32_t r = 0; r < 8; r++)
-            for (uint32_t q = 0; q < 4; q++) {
-                const float f[4] = {(float)M[8 * r + 2 * q].real(), (float)M[8 * r + 2 * q].imag(),
-                                    (float)M[8 * r + 2 * q + 1].real(), (float)M[8 * r + 2 * q + 1].imag()};
-                cd slot;
-                static_assert(sizeof(slot) == sizeof(f), "four floats per complex slot");
-                memcpy((void *)&slot, f, sizeof(f));
-                s->coef.push_back(slot);
+        d.omask = b.omask;
+        // one matrix, or two when some ops are conditioned on outer qubits
+        // (variant 1 for tiles whose outer qubits meet the condition)
+        const uint32_t nvar = b.omask ? 2 : 1;
+        std::vector<cd> M(64 * nvar);
+        for (uint32_t var = 0; var < nvar; var++)
+            for (uint32_t c = 0; c < 8; c++) {
+                cd v[8] = {};
+                v[c] = 1;
+                for (const BOpD &o : b.ops) apply_local(*s, o, pos, 3, v, var == 1);
+                for (uint32_t r = 0; r < 8; r++) M[64 * var + 8 * r + c] = v[r];
             }
+        d.coef = (uint32_t)s->coef.size();
+        s->coef.insert(s->coef.end(), M.begin(), M.end());
+        // complex64: the kernel's FP32 copy per variant (four floats per slot: row r,
+        // columns 2q and 2q + 1)
+        d.a = (uint32_t)s->coef.size();
+        for (uint32_t var = 0; var < nvar && fp32_copy; var++)
+            for (uint32_t r = 0; r < 8; r++)
+                for (uint32_t q = 0; q < 4; q++) {
+                    const cd *m = &M[64 * var + 8 * r + 2 * q];
+                    const float f[4] = {(float)m[0].real(), (float)m[0].imag(), (float)m[1].real(), (float)m[1].imag()};
+                    cd slot;
+                    static_assert(sizeof(slot) == sizeof(f), "four floats per complex slot");
+                    memcpy((void *)&slot, f, sizeof(f));
+                    s->coef.push_back(slot);
+                }
         out.push_back(d);
     };
     auto close_if = [&](uint32_t m) {
@@ -405,13 +442,20 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
             if (o.kind == BK_DM) m = o.lmask;
             close_if(m);
             out.push_back(o);
+            prev.valid = false;
             continue;
         }
         uint32_t u = sup;
+        uint64_t om = o.omask;
+        bool ok = true;
         for (const Blk &b : open)
-            if (b.mask & sup) u |= b.mask;
-        if (__builtin_popcount(u) <= 3) {
-            Blk nb{u, {}};
+            if (b.mask & sup) {
+                u |= b.mask;
+                ok = ok && agree(om, b.omask);
+                om |= b.omask;
+            }
+        if (ok && __builtin_popcount(u) <= 3) {
+            Blk nb{u, {}, om};
             for (size_t i = 0; i < open.size();) {
                 if (open[i].mask & sup) {
                     nb.ops.insert(nb.ops.end(), open[i].ops.begin(), open[i].ops.end());
@@ -424,7 +468,7 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in) {
             open.push_back(std::move(nb));
         } else {
             close_if(sup);
-            open.push_back(Blk{sup, {o}});
+            open.push_back(Blk{sup, {o}, o.omask});
         }
     }
     for (const Blk &b : open) emit(b);
@@ -774,14 +818,15 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         for (size_t i = 0; i + 1 < P->steps.size(); i++)
             if (P->steps[i].modifies) P->steps[i].flags |= BF_STORE;
     }
-    const bool fuse = prec == NCSV_C64 && block_fusion_enabled();
+    const bool fuse = block_fusion_enabled();
+    const bool f32 = prec == NCSV_C64;
     for (StepH &st : P->steps) {
         if (fuse) {
             std::vector<BOpD> clean;
             for (const BOpD &o : st.ops)
                 if (o.kind != BK_NSLOT) clean.push_back(o);
-            st.gops = group_ops(&st, fuse_blocks(&st, st.ops), false);
-            st.gopsc = group_ops(&st, fuse_blocks(&st, clean), true);
+            st.gops = group_ops(&st, fuse_blocks(&st, st.ops, f32), false);
+            st.gopsc = group_ops(&st, fuse_blocks(&st, clean, f32), true);
         } else {
             st.gops = group_ops(&st, st.ops, false);
             st.gopsc = group_ops(&st, st.ops, true);
@@ -799,6 +844,12 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             for (int k = 0; k < 14; k++)
                 if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
             fprintf(stderr, "\n");
+            if (atoi(getenv("NCSV_BATCH_DUMP")) >= 2) {
+                // the slot-free op list: kind, positions, in-tile and outer condition masks
+                for (const BOpD &o : st.gopsc)
+                    fprintf(stderr, "  %s p=%u,%u,%u l=%x o=%llx\n", nm[o.kind], o.p0, o.p1, o.p2, o.lmask,
+                            (unsigned long long)o.omask);
+            }
         }
     }
     for (uint32_t s = 0; s < P->steps.size(); s++) {
@@ -823,7 +874,7 @@ struct ncsv_batch {
     uint32_t tile_bits = 0;  // 0: the largest tile for the precision
     int blocks = 0;          // 0: every resident workgroup
     int low_bits = -1;       // fixed low tile qubits (-1: 5 for complex64, 4 for complex128)
-    int items_lg = -1;       // log2 of the most tiles per queue item (-1: 3)
+    int items_lg = -1;       // log2 of the most tiles per queue item (-1: 5)
     struct Entry {
         const ncsv_circuit *c = nullptr;
         std::vector<uint64_t> x, z;
@@ -971,6 +1022,11 @@ Tables flatten(ncsv_batch *b) {
                 if (i < s.L && i < 16) d.tpos[i] = s.qt[i];
                 if (i >= s.L) d.omask |= 1u << s.qt[i];
             }
+            if (!s.slots.empty()) {
+                const auto mm = std::minmax_element(s.slots.begin(), s.slots.end());
+                d.slot_lo = *mm.first;
+                d.slot_n = *mm.second - *mm.first + 1;
+            }
             const uint32_t c0 = (uint32_t)(t.coef.size() / 2), p0 = (uint32_t)t.paulis.size(),
                            g0 = (uint32_t)t.mgroups.size();
             auto put = [&](const std::vector<BOpD> &ops) {
@@ -1081,7 +1137,7 @@ void fill_draws(const Program &P, const double *u, uint64_t *seed, const uint8_t
 Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *uniforms,
                  const std::vector<size_t> &draw_off, const uint64_t *seeds, const uint8_t *forced) {
     // tiles per queue item: fewer, longer items mean fewer device-scope releases
-    uint32_t lg_tiles_per_item = b->items_lg >= 0 ? (uint32_t)b->items_lg : 3u;
+    uint32_t lg_tiles_per_item = b->items_lg >= 0 ? (uint32_t)b->items_lg : 5u;
     if (const char *e = getenv("NCSV_BATCH_TILES_PER_ITEM")) {
         const int v = atoi(e);
         lg_tiles_per_item = 0;
@@ -1372,6 +1428,7 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                     }
                     case BK_U3: {
                         const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
+                        const uint32_t vo = (op.omask && (base & op.omask) == op.omask) ? 64 : 0;
                         for (uint32_t x = 0; x < NL; x++) {
                             if (x & (pb[0] | pb[1] | pb[2])) continue;
                             uint32_t xs[8];
@@ -1382,7 +1439,7 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                             }
                             for (uint32_t r = 0; r < 8; r++) {
                                 cd acc = 0;
-                                for (uint32_t c = 0; c < 8; c++) acc += C(8 * r + c) * v[c];
+                                for (uint32_t c = 0; c < 8; c++) acc += C(vo + 8 * r + c) * v[c];
                                 tl[xs[r]] = acc;
                             }
                         }

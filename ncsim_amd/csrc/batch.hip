@@ -11,7 +11,9 @@ namespace kern {
 template <typename T> struct V2;
 template <> struct V2<float> { using type = float2; };
 template <> struct V2<double> { using type = double2; };
-typedef float F4 __attribute__((ext_vector_type(4)));
+template <typename T> struct M4;
+template <> struct M4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct M4<double> { typedef double type __attribute__((ext_vector_type(4))); };
 }  // namespace kern
 }  // namespace batch
 }  // namespace ncsv
@@ -34,6 +36,7 @@ typedef float F4 __attribute__((ext_vector_type(4)));
 #define NCSV_LANE0_STORE(dst, v) ((dst) = ::ncsv::batch::kern::uniform64(v))
 #define NCSV_OPAQUE(x) asm volatile("" : "+v"(x))
 #define NCSV_MFMA16(a, b, acc) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (acc), 0, 0, 0)
+#define NCSV_MFMA16D(a, b, acc) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (acc), 0, 0, 0)
 
 namespace ncsv {
 namespace batch {

@@ -43,7 +43,8 @@ enum BOpKind : uint8_t {
     BK_COLLAPSE = 10,  // group c decided from b tiles of cell weights at a; collapse, resets
     BK_EXPECT = 11, // Paulis [c, c+b) to part[a + 2b tile + 2k]
     BK_G3 = 12,     // k1 (2 or 3) unconditional 2x2s on distinct positions p0, p1, p2 in one pass
-    BK_U3 = 13,     // dense 8x8 on positions p0 < p1 < p2 (coef: 64, row-major; complex64 on FP32 MFMA)
+    BK_U3 = 13,     // dense 8x8 on distinct positions p0, p1, p2 (coef: 64, row-major, then 64 more for tiles
+                    // meeting omask when omask != 0; a: the FP32 copies in complex64)
 };
 
 struct BOpD {
@@ -84,8 +85,8 @@ struct BStepD {
     uint32_t flags;
     uint32_t qoff;   // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
     uint32_t omask;  // the outer qubits (tile number bit b <-> the b-th lowest)
-    uint32_t pad;
-    uint8_t tpos[16];  // the qubits of tile positions 0..min(L, 16)-1
+    uint32_t slot_lo, slot_n;  // the noise slots of this step lie in [slot_lo, slot_lo + slot_n)
+    uint8_t tpos[16];          // the qubits of tile positions 0..min(L, 16)-1
 };
 
 struct BEntryD {

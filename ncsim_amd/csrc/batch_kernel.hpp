@@ -11,7 +11,8 @@
 //   (dst = lane 0's v), NCSV_OPAQUE(x) (x unknown to the optimizer from
 //   here on), NCSV_MFMA16(a, b, acc) (one v_mfma_f32_16x16x4_f32: lane l
 //   holds A[l % 16][l / 16] and B[l / 16][l % 16]; acc[v] is D[4 (l / 16) + v][l % 16]),
-//   V2<T>::type (float2 / double2), F4 (4 floats).
+//   NCSV_MFMA16D(a, b, acc) (v_mfma_f64_16x16x4_f64: A and B as above, acc[v] is
+//   D[4 v + l / 16][l % 16]), V2<T>::type (float2 / double2), M4<T>::type (4 floats / doubles).
 //
 // The tile sits in LDS at swizzled slots: slot(x) = x ^ f(x >> S), an
 // XOR-linear bijection that spreads the strided patterns of pair, quad and
@@ -31,6 +32,7 @@ constexpr uint32_t BT = 512;        // threads per workgroup
 constexpr uint32_t NW = BT / 64;    // wavefronts per workgroup
 constexpr uint32_t LOB = 9;         // log2(BT): tile bits taken from the thread index
 constexpr uint32_t CMAX = 18;       // values per block reduction (9 Paulis)
+constexpr uint32_t SMAX = 4096;     // noise codes staged in LDS per record
 
 template <typename V> NCSV_DEV inline V cm(V a, V b) {
     V r;
@@ -75,6 +77,10 @@ NCSV_DEV inline uint32_t deposit(uint32_t j, uint32_t mask, uint32_t ones) {
 template <uint32_t N> struct KTag {
     static constexpr uint32_t value = N;
 };
+
+// One 16x16x4 MFMA of the calling wavefront in the element type.
+NCSV_DEV inline M4<float>::type mfma16(float a, float b, M4<float>::type acc) { return NCSV_MFMA16(a, b, acc); }
+NCSV_DEV inline M4<double>::type mfma16(double a, double b, M4<double>::type acc) { return NCSV_MFMA16D(a, b, acc); }
 
 NCSV_DEV inline uint32_t par32(uint32_t v) { return (uint32_t)__builtin_popcount(v) & 1u; }
 NCSV_DEV inline uint32_t par64(uint64_t v) { return (uint32_t)__builtin_popcountll(v) & 1u; }
@@ -159,6 +165,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
     NCSV_SHARED double red[NW * CMAX];
     NCSV_SHARED double bc[CMAX];
     NCSV_SHARED unsigned long long s_claim;
+    NCSV_SHARED uint8_t scode[SMAX];  // the record's noise codes (steps with an error drawn)
     const uint32_t tid = NCSV_TID;
     // Queue traffic (claim, wait, completion) is done by the whole first
     // wavefront under a wave-uniform branch, and the claim is read back as a
@@ -226,6 +233,12 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
         };
         const uint32_t op0 = (rec.flags & RF_ERR) ? st.op0 : st.op0c;
         const uint32_t op1 = op0 + ((rec.flags & RF_ERR) ? st.nops : st.nopsc);
+        if (rec.flags & RF_ERR) {
+            // this entry's noise codes for the step's slots (read after the first
+            // tile's load barrier; the previous record ended with a barrier)
+            const uint32_t ns = st.slot_n < SMAX ? st.slot_n : SMAX;
+            for (uint32_t t = tid; t < ns; t += BT) scode[t] = errs[en.err0 + st.slot_lo + t];
+        }
         // each thread reads back only the LDS slots it wrote, so consecutive
         // tiles need no barrier between the store of one and the load of the next
         for (uint32_t ti = 0; ti < ntl; ti++) {
@@ -383,72 +396,101 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     break;
                 }
                 case BK_U3: {
-                    // dense 8x8 on positions p0, p1, p2 (local bit i <-> p_i), complex64 only:
-                    // per 16 groups of 8 amplitudes, D (16 x 16) = A (16 x 16) x B (16 x 16) in
-                    // four FP32 MFMAs, A the real form of the 8x8.  Row or column 4q + c of a
-                    // group <-> amplitude 2q + (c >> 1), real part for even c, imaginary for odd
-                    // c, so lane l (group l % 16, q = l / 16) reads and writes amplitudes 2q and
-                    // 2q + 1 of its group and nothing else.
+                    // dense 8x8 on positions p0, p1, p2 (local bit i <-> p_i): per 16 groups
+                    // of 8 amplitudes, D (16 x 16) = A (16 x 16) x B (16 x 16) in four MFMAs
+                    // (FP32 for complex64, FP64 for complex128), A the real form of the 8x8.
+                    // Column 4 kb + q of B <-> amplitude 2q + (kb >> 1), real part for even
+                    // kb, imaginary for odd; lane l (group l % 16, q = l / 16) supplies and
+                    // receives amplitudes 2q and 2q + 1 of its group and nothing else.
+                    // A[i][k] sits in lane i + 16 (k % 4), register k / 4 (both sizes); the
+                    // row i of output register v in lane l is 4 q + v (FP32) or 4 v + q
+                    // (FP64), so A's rows are ordered to match.
+                    const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
+                    const uint32_t mask = pb[0] | pb[1] | pb[2];
+                    const uint32_t lane = tid & 63, q = lane >> 4, wv = tid >> 6;
+                    const uint32_t i = lane & 15;
+                    const uint32_t ai = sizeof(T) == 4 ? 2 * (i >> 2) + ((i >> 1) & 1) : 2 * (i & 3) + (i >> 3);
+                    const uint32_t ci = sizeof(T) == 4 ? (i & 1) : ((i >> 2) & 1);
+                    // two matrices when ops of the block are conditioned on outer qubits:
+                    // the second for tiles that meet the condition
+                    const uint32_t var = (op.omask && (base & (uint32_t)op.omask) == (uint32_t)op.omask) ? 1u : 0u;
+                    T u0r, u0i, u1r, u1i;  // U[ai][2q], U[ai][2q + 1]
                     if constexpr (sizeof(T) == 4) {
-                        const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
-                        const uint32_t mask = pb[0] | pb[1] | pb[2];
-                        const uint32_t lane = tid & 63, q = lane >> 4, wv = tid >> 6;
-                        // A[lane % 16][4 kb + q] from the FP32 copy of the block (row-major
-                        // complex pairs at op.a): the lane's row ai, columns 2q and 2q + 1
-                        const uint32_t i = lane & 15, ai = 2 * (i >> 2) + ((i >> 1) & 1), ci = i & 1;
-                        const float *ur = (const float *)(cf + 2 * op.a) + 16 * ai + 4 * q;
-                        const float u0r = ur[0], u0i = ur[1], u1r = ur[2], u1i = ur[3];
-                        const float av[4] = {ci ? u0i : u0r, ci ? u0r : -u0i, ci ? u1i : u1r, ci ? u1r : -u1i};
-                        auto off = [&](uint32_t a) {
-                            return sw<V>(((a & 1) ? pb[0] : 0u) | ((a & 2) ? pb[1] : 0u) | ((a & 4) ? pb[2] : 0u));
-                        };
-                        const uint32_t sa0 = off(2 * q), sa1 = off(2 * q + 1);
-                        const uint32_t ngroups = NL >> 3, gl = (lane & 15) + 16 * wv;
-                        auto round = [&](uint32_t sh, bool valid) {
-                            const V in0 = tile[sh ^ sa0], in1 = tile[sh ^ sa1];
-                            F4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                            acc = NCSV_MFMA16(av[0], valid ? in0.x : 0.0f, acc);
-                            acc = NCSV_MFMA16(av[1], valid ? in0.y : 0.0f, acc);
-                            acc = NCSV_MFMA16(av[2], valid ? in1.x : 0.0f, acc);
-                            acc = NCSV_MFMA16(av[3], valid ? in1.y : 0.0f, acc);
-                            if (valid) {
-                                V o0, o1;
-                                o0.x = acc[0];
-                                o0.y = acc[1];
-                                o1.x = acc[2];
-                                o1.y = acc[3];
-                                tile[sh ^ sa0] = o0;
-                                tile[sh ^ sa1] = o1;
-                            }
-                        };
-                        if (ngroups >= 16 * NW) {
-                            // every group valid; round rd covers groups 16 NW rd + gl, its
-                            // uniform part stepped by a masked increment
-                            const uint32_t st = sw<V>(deposit(gl, mask, 0));
-                            const uint32_t fill = mask | deposit(16 * NW - 1, mask, 0);
-                            const uint32_t nrounds = ngroups / (16 * NW);
-                            uint32_t h = 0;
-#pragma unroll 2
-                            for (uint32_t rd = 0; rd < nrounds; rd++) {
-                                round(st ^ sw<V>(h), true);
-                                h = ((h | fill) + 1) & ~fill;
-                            }
-                        } else {
-                            // one round; the same MFMAs in every wavefront
-                            const bool valid = gl < ngroups;
-                            round(sw<V>(deposit(valid ? gl : 0u, mask, 0)), valid);
+                        // the FP32 copy of the block at op.a: row-major complex pairs
+                        const float *ur = (const float *)(cf + 2 * (op.a + 32 * var)) + 16 * ai + 4 * q;
+                        u0r = ur[0];
+                        u0i = ur[1];
+                        u1r = ur[2];
+                        u1i = ur[3];
+                    } else {
+                        const double *ur = cf + 2 * (op.coef + 64 * var + 8 * ai + 2 * q);
+                        u0r = ur[0];
+                        u0i = ur[1];
+                        u1r = ur[2];
+                        u1i = ur[3];
+                    }
+                    const T av[4] = {ci ? u0i : u0r, ci ? u0r : -u0i, ci ? u1i : u1r, ci ? u1r : -u1i};
+                    auto off = [&](uint32_t a) {
+                        return sw<V>(((a & 1) ? pb[0] : 0u) | ((a & 2) ? pb[1] : 0u) | ((a & 4) ? pb[2] : 0u));
+                    };
+                    const uint32_t sa0 = off(2 * q), sa1 = off(2 * q + 1);
+                    const uint32_t ngroups = NL >> 3, gl = (lane & 15) + 16 * wv;
+                    using A4 = typename M4<T>::type;
+                    auto mult = [&](V in0, V in1) {
+                        A4 acc = {(T)0, (T)0, (T)0, (T)0};
+                        acc = mfma16(av[0], in0.x, acc);
+                        acc = mfma16(av[1], in0.y, acc);
+                        acc = mfma16(av[2], in1.x, acc);
+                        return mfma16(av[3], in1.y, acc);
+                    };
+                    auto put = [&](uint32_t sh, const A4 &acc) {
+                        V o0, o1;
+                        o0.x = acc[0];
+                        o0.y = acc[1];
+                        o1.x = acc[2];
+                        o1.y = acc[3];
+                        tile[sh ^ sa0] = o0;
+                        tile[sh ^ sa1] = o1;
+                    };
+                    if (ngroups >= 32 * NW) {
+                        // every group valid, two rounds at a time (two independent MFMA
+                        // chains); round rd covers groups 16 NW rd + gl, its uniform part
+                        // stepped by a masked increment
+                        const uint32_t st = sw<V>(deposit(gl, mask, 0));
+                        const uint32_t fill = mask | deposit(16 * NW - 1, mask, 0);
+                        const uint32_t npairs = ngroups / (32 * NW);
+                        uint32_t h = 0;
+                        for (uint32_t rp = 0; rp < npairs; rp++) {
+                            const uint32_t h1 = ((h | fill) + 1) & ~fill;
+                            const uint32_t sh0 = st ^ sw<V>(h), sh1 = st ^ sw<V>(h1);
+                            const V a0 = tile[sh0 ^ sa0], a1 = tile[sh0 ^ sa1];
+                            const V b0 = tile[sh1 ^ sa0], b1 = tile[sh1 ^ sa1];
+                            const A4 ra = mult(a0, a1), rb = mult(b0, b1);
+                            put(sh0, ra);
+                            put(sh1, rb);
+                            h = ((h1 | fill) + 1) & ~fill;
                         }
-                    }  // complex128 programs never hold this op
+                    } else {
+                        // one round; the same MFMAs in every wavefront
+                        const bool valid = gl < ngroups;
+                        const uint32_t sh = sw<V>(deposit(valid ? gl : 0u, mask, 0));
+                        V in0 = tile[sh ^ sa0], in1 = tile[sh ^ sa1];
+                        if (!valid) {
+                            in0.x = in0.y = in1.x = in1.y = (T)0;
+                        }
+                        const A4 r = mult(in0, in1);
+                        if (valid) put(sh, r);
+                    }
                     break;
                 }
                 case BK_DM: {
-                    if ((base & op.omask) != op.omask) break;
+                    if ((base & op.omask) != op.omask) continue;  // not this tile: no barrier
                     const V ph = coef<V>(cf, op.coef);
                     visit<V>(NL, tid, op.lmask, op.lmask, [&](uint32_t, uint32_t s) { tile[s] = cm(ph, tile[s]); });
                     break;
                 }
                 case BK_PERM1: {
-                    if ((base & op.omask) != op.omask) break;
+                    if ((base & op.omask) != op.omask) continue;  // not this tile: no barrier
                     const uint32_t bp = 1u << op.p0, sb = sw<V>(bp);
                     const uint32_t k0 = op.k0, k1 = op.k1;
                     visit<V>(NL, tid, op.lmask | bp, op.lmask, [&](uint32_t, uint32_t s0) {
@@ -470,8 +512,9 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     break;
                 }
                 case BK_NSLOT: {
-                    const uint32_t code = errs[en.err0 + op.a];
-                    if (!code) break;
+                    const uint32_t si = op.a - st.slot_lo;
+                    const uint32_t code = NCSV_UNIFORM32(si < SMAX ? (uint32_t)scode[si] : (uint32_t)errs[en.err0 + op.a]);
+                    if (!code) continue;  // no error at this site: the tile is unchanged, no barrier
                     const uint32_t sb = sw<V>(1u << op.p0);
                     visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;

@@ -26,10 +26,13 @@ struct hd2 {
 template <typename T> struct V2;
 template <> struct V2<float> { using type = hf2; };
 template <> struct V2<double> { using type = hd2; };
-struct F4 {
-    float v[4];
-    float &operator[](int i) { return v[i]; }
-    float operator[](int i) const { return v[i]; }
+template <typename T> struct Q4 {
+    T v[4];
+    T &operator[](int i) { return v[i]; }
+    T operator[](int i) const { return v[i]; }
+};
+template <typename T> struct M4 {
+    using type = Q4<T>;
 };
 }  // namespace kern
 
@@ -54,18 +57,19 @@ struct Barrier {
 thread_local unsigned tid = 0;
 Barrier *bar = nullptr;
 double xbuf[1024];
-float ma[1024], mb[1024];
-// v_mfma_f32_16x16x4_f32 of the calling thread's wavefront (lane l: A[l % 16][l / 16],
-// B[l / 16][l % 16], acc[v] += D[4 (l / 16) + v][l % 16]); FP32 products and sums.
-kern::F4 mfma16(float a, float b, kern::F4 acc) {
+double ma[1024], mb[1024];
+// The 16x16x4 MFMAs of the calling thread's wavefront: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; acc[v] += D[i][l % 16] with i = 4 (l / 16) + v for FP32
+// (v_mfma_f32_16x16x4_f32) and i = 4 v + l / 16 for FP64 (v_mfma_f64_16x16x4_f64).
+template <typename T> kern::Q4<T> mfma16(T a, T b, kern::Q4<T> acc) {
     ma[tid] = a;
     mb[tid] = b;
     bar->wait();
     const unsigned w0 = tid & ~63u, l = tid & 63u;
     for (unsigned v = 0; v < 4; v++) {
-        const unsigned i = 4 * (l / 16) + v, j = l % 16;
-        float d = 0.0f;
-        for (unsigned k = 0; k < 4; k++) d += ma[w0 + i + 16 * k] * mb[w0 + j + 16 * k];
+        const unsigned i = sizeof(T) == 4 ? 4 * (l / 16) + v : 4 * v + l / 16, j = l % 16;
+        T d = (T)0;
+        for (unsigned k = 0; k < 4; k++) d += (T)ma[w0 + i + 16 * k] * (T)mb[w0 + j + 16 * k];
         acc[v] += d;
     }
     bar->wait();
@@ -98,7 +102,8 @@ double shfl_xor(double x, unsigned mask) {
 #define NCSV_UNIFORM64(v) (v)
 #define NCSV_UNIFORM32(v) (v)
 #define NCSV_OPAQUE(x) ((void)0)
-#define NCSV_MFMA16(a, b, acc) ::ncsv::batch::sim::mfma16((a), (b), (acc))
+#define NCSV_MFMA16(a, b, acc) ::ncsv::batch::sim::mfma16<float>((a), (b), (acc))
+#define NCSV_MFMA16D(a, b, acc) ::ncsv::batch::sim::mfma16<double>((a), (b), (acc))
 #define NCSV_LANE0_STORE(dst, v) \
     do {                         \
         if (NCSV_TID == 0) (dst) = (v); \

@@ -240,20 +240,21 @@ def _planned_kinds(capfd, b):
     return {k.split("=")[0] for line in err.splitlines() if line.startswith("step") for k in line.split(":")[1].split()}
 
 
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
 @pytest.mark.parametrize("tile_bits", [0, 6, 8])
-def test_dense_blocks_match_oracle(tile_bits, capfd, monkeypatch):
-    """Complex64 programs fuse ops on up to three tile positions into dense
-    8x8 blocks (U3); the same programs without them (NCSV_BATCH_MFMA=0) and
+def test_dense_blocks_match_oracle(precision, tile_bits, capfd, monkeypatch):
+    """Programs fuse ops on up to three tile positions into dense 8x8 blocks
+    (U3, run on MFMA); the same programs without them (NCSV_BATCH_MFMA=0) and
     the oracle agree with them (FP64 emulation of the step lists)."""
     monkeypatch.setenv("NCSV_BATCH_DUMP", "1")
     texts = [random_circuit(n, d, s) for n, d, s in CASES]
     paulis = [random_paulis(n, 4, s) for n, d, s in CASES]
-    b = _build(texts, paulis, nc.C64, tile_bits)
+    b = _build(texts, paulis, precision, tile_bits)
     assert "U3" in _planned_kinds(capfd, b)
     us = _draws(b, 12)
     b.emulate(uniforms=us)
     monkeypatch.setenv("NCSV_BATCH_MFMA", "0")
-    b0 = _build(texts, paulis, nc.C64, tile_bits)
+    b0 = _build(texts, paulis, precision, tile_bits)
     assert "U3" not in _planned_kinds(capfd, b0)
     b0.emulate(uniforms=us)
     for i, (t, ps) in enumerate(zip(texts, paulis)):
@@ -268,22 +269,28 @@ def test_dense_blocks_match_oracle(tile_bits, capfd, monkeypatch):
 
 def test_dense_blocks_fuse_brickwork(capfd, monkeypatch):
     """A 16-qubit brickwork of SU(2)s and CXs on 13-qubit tiles: blocks cut
-    the passes per step at least in half."""
+    the passes per step at least in half, and CXs controlled by qubits
+    outside the tile join blocks as a second matrix for the tiles where the
+    control is set."""
     from ncsim_amd import workloads as W
-    monkeypatch.setenv("NCSV_BATCH_DUMP", "1")
+    monkeypatch.setenv("NCSV_BATCH_DUMP", "2")
     c = nc.Circuit.parse(W.noisy_brickwork(16, 12, 3, p=1e-3, every=100))
 
-    def passes(env):
+    def plan(env):
         monkeypatch.setenv("NCSV_BATCH_MFMA", env)
         b = nc.HipBatch(nc.C64)
         b.set_tile_bits(13)
         b.add(c, [])
         capfd.readouterr()
         b.plan()
-        err = capfd.readouterr().err
-        return sum(int(line.split("clean=")[1].split()[0]) for line in err.splitlines() if line.startswith("step"))
+        return capfd.readouterr().err.splitlines()
 
-    assert 2 * passes("1") <= passes("0")
+    def passes(lines):
+        return sum(int(line.split("clean=")[1].split()[0]) for line in lines if line.startswith("step"))
+
+    fused = plan("1")
+    assert 2 * passes(fused) <= passes(plan("0"))
+    assert any(line.strip().startswith("U3") and not line.endswith("o=0") for line in fused)
 
 
 @pytest.mark.parametrize("precision", [nc.C128, nc.C64])
