@@ -367,8 +367,9 @@ def main():
     gates = W.count_gates(text)
     paulis = [H.Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 64, 4)]
     circ = H.Circuit.parse(text)
-    log(f"bench: n={n} depth={depth} gates={gates} paulis=64 fusion={args.fusion}")
-    s = H.HipStateVector.zero_state(n, H.C128)
+    prec = H.C64 if args.precision == "c64" else H.C128
+    log(f"bench: n={n} depth={depth} gates={gates} paulis=64 fusion={args.fusion} precision={args.precision}")
+    s = H.HipStateVector.zero_state(n, prec)
     s.set_fusion(args.fusion)
 
     def step():
@@ -403,7 +404,8 @@ def main():
     achieved = byts / launches / avg_s
     log("bench: kernels " + ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(stats.items())))
     roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(n, depth, args.fusion, name),
+                "frac": achieved / HBM_PEAK,
+                "traffic": measured_traffic(n, depth, args.fusion, name) if prec == H.C128 else None,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                 "algorithmic_bytes": byts / launches, "launches": launches,
                 "avg_ms": tot_ms / launches, "sweeps_per_step": sweeps}
@@ -411,9 +413,10 @@ def main():
     out = {
         "metric": "amp-updates/s", "value": value, "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128 (f64)", "data": "synthetic (seeded circuit)",
+        "dtype": "c128 (f64)" if prec == H.C128 else "c64 (f32, f64 reductions)", "data": "synthetic (seeded circuit)",
         "config": {"workload": f"rotation brickwork {n} qubits depth {depth} + 64 Pauli expectations",
-                   "qubits": n, "depth": depth, "gates": gates, "fusion": args.fusion, "precision": "complex128"},
+                   "qubits": n, "depth": depth, "gates": gates, "fusion": args.fusion,
+                   "precision": "complex128" if prec == H.C128 else "complex64"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
