@@ -10,6 +10,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <future>
@@ -36,6 +37,7 @@ struct Gen {
 
     bool literal = false;  // coefficients as constants in the code instead of loads from c[]
     bool dbl = true;
+    double scale = 1.0;  // common factor taken out of the real-form 2x2s, applied at the store
 
     uint32_t push(double v) {
         coef->push_back(v);
@@ -154,61 +156,101 @@ void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
 
 bool is_exact(cd c, double re, double im) { return c.real() == re && c.imag() == im; }
 
+// A real 2x2 r as f * q with as many entries of q equal to +-1 as one common
+// factor allows (two for rotations and reflections, four for H), |f| largest
+// among the choices that reach that count, so |q| <= 1 for those.  The
+// factor is multiplied into the sweep's scale and applied once at the store;
+// a +-1 entry costs an add instead of a multiply-add.
+struct Factored {
+    double f = 1.0;
+    double q[4];
+};
+Factored factor_2x2(const double r[4]) {
+    auto units = [](const double *q) {
+        int u = 0;
+        for (int j = 0; j < 4; j++) u += (q[j] == 1.0 || q[j] == -1.0);
+        return u;
+    };
+    Factored best;
+    for (int j = 0; j < 4; j++) best.q[j] = r[j];
+    int bu = units(best.q);
+    for (int i = 0; i < 4; i++) {
+        if (r[i] == 0.0) continue;
+        Factored c;
+        c.f = r[i];
+        for (int j = 0; j < 4; j++) c.q[j] = r[j] / c.f;
+        const int u = units(c.q);
+        if (u > bu || (u == bu && u > 0 && std::fabs(c.f) > std::fabs(best.f))) {
+            best = c;
+            bu = u;
+        }
+    }
+    return best;
+}
+
+// q * v as a term of a sum: "v", "-v" or "C*v"
+std::string term(Gen &g, double q, const std::string &v, bool first) {
+    if (q == 1.0) return (first ? "" : " + ") + v;
+    if (q == -1.0) return (first ? "-" : " - ") + v;
+    return (first ? "" : " + ") + g.C(g.push(q)) + "*" + v;
+}
+
 void emit_op(Gen &g, const TileOpH &op, uint32_t cur) {
     auto &o = g.o;
     const uint32_t NR = g.NR;
     switch (op.kind) {
-    case TK_U1:
     case TK_U1R:
     case TK_U1X: {
+        // real form: U1R [[r0, r1], [r2, r3]]; U1X [[r0, i r1], [i r2, r3]]
         const uint32_t J = op.j0;
-        uint32_t k0;
-        if (op.kind == TK_U1) {
-            k0 = g.push(op.m[0].real());
-            for (int t = 0; t < 4; t++) {
-                if (t) g.push(op.m[t].real());
-                g.push(op.m[t].imag());
-            }
-        } else if (op.kind == TK_U1R) {
-            k0 = g.push(op.m[0].real());
-            g.push(op.m[1].real());
-            g.push(op.m[2].real());
-            g.push(op.m[3].real());
-        } else {
-            k0 = g.push(op.m[0].real());
-            g.push(op.m[1].imag());
-            g.push(op.m[2].imag());
-            g.push(op.m[3].real());
-        }
+        const bool xk = op.kind == TK_U1X;
+        const double r[4] = {op.m[0].real(), xk ? op.m[1].imag() : op.m[1].real(), xk ? op.m[2].imag() : op.m[2].real(),
+                             op.m[3].real()};
+        const Factored fq = factor_2x2(r);
+        g.scale *= fq.f;
         o << "  {\n";
         for (uint32_t i = 0; i < NR; i++) {
             if (i & (1u << J)) continue;
             const std::string a = g.var(i), b = g.var(i | (1u << J));
             o << "    { const V x = " << a << ", y = " << b << ";\n";
-            if (op.kind == TK_U1) {
-                // m = [a0 a1 | b0 b1 | c0 c1 | d0 d1] at k0..k0+7
-                auto c = [&](int t) { return g.C(k0 + t); };
-                o << "      " << a << ".x = " << c(0) << "*x.x - " << c(1) << "*x.y + " << c(2) << "*y.x - " << c(3)
-                  << "*y.y;\n";
-                o << "      " << a << ".y = " << c(0) << "*x.y + " << c(1) << "*x.x + " << c(2) << "*y.y + " << c(3)
-                  << "*y.x;\n";
-                o << "      " << b << ".x = " << c(4) << "*x.x - " << c(5) << "*x.y + " << c(6) << "*y.x - " << c(7)
-                  << "*y.y;\n";
-                o << "      " << b << ".y = " << c(4) << "*x.y + " << c(5) << "*x.x + " << c(6) << "*y.y + " << c(7)
-                  << "*y.x; }\n";
-            } else if (op.kind == TK_U1R) {
-                auto c = [&](int t) { return g.C(k0 + t); };
-                o << "      " << a << ".x = " << c(0) << "*x.x + " << c(1) << "*y.x; " << a << ".y = " << c(0)
-                  << "*x.y + " << c(1) << "*y.y;\n";
-                o << "      " << b << ".x = " << c(2) << "*x.x + " << c(3) << "*y.x; " << b << ".y = " << c(2)
-                  << "*x.y + " << c(3) << "*y.y; }\n";
+            if (!xk) {
+                o << "      " << a << ".x = " << term(g, fq.q[0], "x.x", true) << term(g, fq.q[1], "y.x", false) << "; ";
+                o << a << ".y = " << term(g, fq.q[0], "x.y", true) << term(g, fq.q[1], "y.y", false) << ";\n";
+                o << "      " << b << ".x = " << term(g, fq.q[2], "x.x", true) << term(g, fq.q[3], "y.x", false) << "; ";
+                o << b << ".y = " << term(g, fq.q[2], "x.y", true) << term(g, fq.q[3], "y.y", false) << "; }\n";
             } else {
-                auto c = [&](int t) { return g.C(k0 + t); };
-                o << "      " << a << ".x = " << c(0) << "*x.x - " << c(1) << "*y.y; " << a << ".y = " << c(0)
-                  << "*x.y + " << c(1) << "*y.x;\n";
-                o << "      " << b << ".x = " << c(3) << "*y.x - " << c(2) << "*x.y; " << b << ".y = " << c(3)
-                  << "*y.y + " << c(2) << "*x.x; }\n";
+                // a = q0 x + i q1 y, b = i q2 x + q3 y
+                o << "      " << a << ".x = " << term(g, fq.q[0], "x.x", true) << term(g, -fq.q[1], "y.y", false) << "; ";
+                o << a << ".y = " << term(g, fq.q[0], "x.y", true) << term(g, fq.q[1], "y.x", false) << ";\n";
+                o << "      " << b << ".x = " << term(g, fq.q[3], "y.x", true) << term(g, -fq.q[2], "x.y", false) << "; ";
+                o << b << ".y = " << term(g, fq.q[3], "y.y", true) << term(g, fq.q[2], "x.x", false) << "; }\n";
             }
+        }
+        o << "  }\n";
+        break;
+    }
+    case TK_U1: {
+        const uint32_t J = op.j0;
+        // m = [a0 a1 | b0 b1 | c0 c1 | d0 d1] at k0..k0+7
+        const uint32_t k0 = g.push(op.m[0].real());
+        for (int t = 0; t < 4; t++) {
+            if (t) g.push(op.m[t].real());
+            g.push(op.m[t].imag());
+        }
+        auto c = [&](int t) { return g.C(k0 + t); };
+        o << "  {\n";
+        for (uint32_t i = 0; i < NR; i++) {
+            if (i & (1u << J)) continue;
+            const std::string a = g.var(i), b = g.var(i | (1u << J));
+            o << "    { const V x = " << a << ", y = " << b << ";\n";
+            o << "      " << a << ".x = " << c(0) << "*x.x - " << c(1) << "*x.y + " << c(2) << "*y.x - " << c(3)
+              << "*y.y;\n";
+            o << "      " << a << ".y = " << c(0) << "*x.y + " << c(1) << "*x.x + " << c(2) << "*y.y + " << c(3)
+              << "*y.x;\n";
+            o << "      " << b << ".x = " << c(4) << "*x.x - " << c(5) << "*x.y + " << c(6) << "*y.x - " << c(7)
+              << "*y.y;\n";
+            o << "      " << b << ".y = " << c(4) << "*x.y + " << c(5) << "*x.x + " << c(6) << "*y.y + " << c(7)
+              << "*y.x; }\n";
         }
         o << "  }\n";
         break;
@@ -477,6 +519,12 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
         flush();
     }
     if (cur != memr) emit_exchange(g, cur, memr);
+    if (g.scale != 1.0) {
+        // the factors taken out of the real-form 2x2s
+        const std::string S = g.C(g.push(g.scale));
+        for (uint32_t r = 0; r < g.NR; r++)
+            o << "  " << g.var(r) << ".x *= " << S << "; " << g.var(r) << ".y *= " << S << ";\n";
+    }
     for (uint32_t r = 0; r < g.NR; r++) o << "  amp[gm | " << roff(r) << "ull] = " << g.var(r) << ";\n";
     o << "}\n";
     return o.str();
