@@ -38,6 +38,7 @@ struct Gen {
     bool literal = false;  // coefficients as constants in the code instead of loads from c[]
     bool dbl = true;
     double scale = 1.0;  // common factor taken out of the real-form 2x2s, applied at the store
+    bool opaque = false;  // exchange addresses recomputed where used (kernels that loop over tiles)
 
     uint32_t push(double v) {
         coef->push_back(v);
@@ -137,14 +138,16 @@ void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
     auto &o = g.o;
     const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
     const std::vector<uint32_t> ra = positions(from, g.L, true), rb = positions(to, g.L, true);
-    o << "  {\n    const unsigned ta = " << deposit("tid", ta, false) << ";\n";
+    o << "  {\n    unsigned ta = " << deposit("tid", ta, false) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(ta));\n";
     for (uint32_t r = 0; r < g.NR; r++) {
         uint32_t off = 0;
         for (uint32_t j = 0; j < g.R; j++)
             if ((r >> j) & 1) off |= 1u << ra[j];
         o << "    lds[swz(ta | " << off << "u)] = " << g.var(r) << ";\n";
     }
-    o << "    __syncthreads();\n    const unsigned tb = " << deposit("tid", tb, false) << ";\n";
+    o << "    __syncthreads();\n    unsigned tb = " << deposit("tid", tb, false) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(tb));\n";
     for (uint32_t r = 0; r < g.NR; r++) {
         uint32_t off = 0;
         for (uint32_t j = 0; j < g.R; j++)
@@ -540,6 +543,8 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
 // tiles with a grid stride; partial sums go to part[k * gridDim.x + block].
 std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo, bool dbl) {
     Gen g;
+    // each exchange's LDS addresses are computed where it runs, not held across the tile loop
+    g.opaque = true;
     std::vector<double> dummy;
     g.coef = &dummy;
     g.LT = geo.LT;
@@ -623,12 +628,13 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
             }
             o << "  { const bool odd = __popcll(g & " << zrest << "ull) & 1; double tr = 0.0, ti = 0.0;\n";
             auto add = [&](int kk, const std::string &re, const std::string &im) {
-                // (tr, ti) += i^kk (re + i im)
+                // (tr, ti) += i^kk (re + i im); an empty im is zero
+                const bool hi = !im.empty();
                 switch (kk & 3) {
-                case 0: o << "    tr += " << re << "; ti += " << im << ";\n"; break;
-                case 1: o << "    tr -= " << im << "; ti += " << re << ";\n"; break;
-                case 2: o << "    tr -= " << re << "; ti -= " << im << ";\n"; break;
-                default: o << "    tr += " << im << "; ti -= " << re << ";\n"; break;
+                case 0: o << "    tr += " << re << ";" << (hi ? " ti += " + im + ";" : "") << "\n"; break;
+                case 1: o << (hi ? "    tr -= " + im + ";" : "   ") << " ti += " << re << ";\n"; break;
+                case 2: o << "    tr -= " << re << ";" << (hi ? " ti -= " + im + ";" : "") << "\n"; break;
+                default: o << (hi ? "    tr += " + im + ";" : "   ") << " ti -= " << re << ";\n"; break;
                 }
             };
             for (uint32_t r = 0; r < g.NR; r++) {
@@ -638,18 +644,30 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
                 const std::string w = g.var(r), u = g.var(r2);
                 if (!xr) {
                     o << "    { const T x = " << w << ".x, y = " << w << ".y; const double m = (double)x*x + (double)y*y;\n";
-                    add(kr, "m", "0.0");
+                    add(kr, "m", "");
                     o << "    }\n";
                 } else {
+                    // i^kr z + i^kr2 conj(z) with z = conj(psi(c^x)) psi(c) and kr2 - kr in {0, 2}:
+                    // 2 i^kr Re z, or 2 i^(kr+1) Im z; the factor 2 is applied per thread below
                     const int kr2 = (int)(P.ph + 2 * __builtin_popcount(zr & r2)) & 3;
-                    o << "    { const double wx = " << w << ".x, wy = " << w << ".y, ux = " << u << ".x, uy = " << u
-                      << ".y;\n      const double pr = ux*wx + uy*wy, pi = ux*wy - uy*wx;\n";
-                    add(kr, "pr", "pi");
-                    add(kr2, "pr", "(-pi)");
+                    if (kr2 == kr) {
+                        o << "    { const double v = (double)" << u << ".x*" << w << ".x + (double)" << u << ".y*" << w
+                          << ".y;\n";
+                        add(kr, "v", "");
+                    } else {
+                        o << "    { const double v = (double)" << u << ".x*" << w << ".y - (double)" << u << ".y*" << w
+                          << ".x;\n";
+                        add(kr + 1, "v", "");
+                    }
                     o << "    }\n";
                 }
             }
-            o << "    if (odd) { tr = -tr; ti = -ti; }\n    sr" << k << " += tr; si" << k << " += ti; }\n";
+            if (xr)
+                o << "    const double f2 = odd ? -2.0 : 2.0;\n    sr" << k << " += f2*tr; si" << k << " += f2*ti; }\n";
+            else
+                o << "    if (odd) { tr = -tr; ti = -ti; }\n    sr" << k << " += tr; si" << k << " += ti; }\n";
+            // one Pauli at a time: interleaving them raised the registers past two wavefronts per SIMD
+            o << "    __builtin_amdgcn_sched_barrier(0);\n";
         }
         o << "  }\n";
     }
@@ -805,16 +823,18 @@ bool available() {
 // (nothing done) when some Pauli's X part does not fit one tile's register
 // stage.
 // Most Paulis per group kernel: every Pauli holds two FP64 accumulators per
-// thread; complex128 groups of up to 12 ran 14 % faster per pass than 16 on
-// 64 random Paulis at 30 qubits (the same six passes), complex64 ones are best at 16.
-uint32_t group_max(bool dbl) {
+// thread.  On 64 random Paulis at 30 qubits, groups of up to 12 gave the
+// fastest step in both precisions (complex128: six passes of 3.5 ms against
+// 3.7 ms at 16; complex64: six of 2.2 ms against five of 3.0 ms, where 16 Paulis
+// take the kernel to 256 registers and one wavefront per SIMD).
+uint32_t group_max() {
     const char *e = getenv("NCSV_EXPECT_GMAX");
-    return e ? (uint32_t)std::min(16, std::max(1, atoi(e))) : (dbl ? 12u : 16u);
+    return e ? (uint32_t)std::min(16, std::max(1, atoi(e))) : 12u;
 }
 
-bool plan_expect(uint32_t n, const TileGeom &geo, bool dbl, uint32_t count, const uint64_t *x, const uint64_t *z,
+bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
                  const uint32_t *ph, std::vector<ExpGroup> *out_groups) {
-    const uint32_t gmax = group_max(dbl);
+    const uint32_t gmax = group_max();
     const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, L), LT = geo.LT;
     if (n < L) return false;
     ncsv_state fake;
@@ -914,7 +934,7 @@ bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64
     const uint32_t L = geo.L(), LT = geo.LT;
     constexpr uint32_t GMAX = 16;
     std::vector<ExpGroup> groups;
-    if (!plan_expect(s->n, geo, s->prec == NCSV_C128, count, x, z, ph, &groups)) return false;
+    if (!plan_expect(s->n, geo, count, x, z, ph, &groups)) return false;
     const bool dbl = s->prec == NCSV_C128;
     std::vector<void *> kern;
     for (auto &eg : groups) kern.push_back(prepare_named(generate_expect(eg, s->n, geo, dbl), "ncsv_expect"));
@@ -953,7 +973,7 @@ ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t coun
         using namespace ncsv;
         const TileGeom geo = dev::tile_geom(prec);
         std::vector<jit::ExpGroup> groups;
-        if (!jit::plan_expect(n, geo, prec == NCSV_C128, count, x, z, ph, &groups)) return NCSV_UNSUPPORTED;
+        if (!jit::plan_expect(n, geo, count, x, z, ph, &groups)) return NCSV_UNSUPPORTED;
         for (auto &eg : groups) {
             std::string log;
             if (jit::compile_to_code(jit::generate_expect(eg, n, geo, prec == NCSV_C128), arch ? arch : "gfx950", &log)

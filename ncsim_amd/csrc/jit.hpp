@@ -43,7 +43,7 @@ void *prepare_named(const std::string &src, const char *fname);
 void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_t grid, double *dpart);
 // <P_k> for all k through specialized group kernels; false when some X part
 // does not fit a tile's register stage (nothing done then).
-bool plan_expect(uint32_t n, const TileGeom &geo, bool dbl, uint32_t count, const uint64_t *x, const uint64_t *z,
+bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
                  const uint32_t *ph, std::vector<ExpGroup> *out_groups);
 bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
                   double *out);
