@@ -127,17 +127,19 @@ def run_qft(args):
 def run_batch(args):
     """Batched form: `--circuits` uneven circuits (10-22 qubits, depth
     10-120, H/S/T/SQRT_X/RZ layers, CX matchings, CCZ with p = 0.05) with 8
-    Paulis each, all in one persistent launch per run.  Compared with
-    per-circuit launches (each circuit on its own state through the fused
-    single-state path) over every `--stride`-th circuit; the same for the
-    subset of 10-14 qubits.  value = circuits/s of the batched form on the
-    whole set."""
+    Paulis each, all in one persistent launch per run, complex64 by default.
+    Compared with per-circuit launches in the same precision (each circuit
+    on its own state through the fused single-state path) over every
+    `--stride`-th circuit; the same for the subset of 10-14 qubits.  In
+    complex64 each sampled circuit is also run in complex128 and its
+    expectations compared with the batch's.  value = circuits/s of the
+    batched form on the whole set."""
     import numpy as np
 
     import ncsim_amd as H
     from ncsim_amd import workloads as W
 
-    prec = H.C64 if args.precision == "c64" else H.C128
+    prec = H.C128 if args.precision == "c128" else H.C64
     specs = W.batch_circuits(args.circuits, 11)
     circs = [H.Circuit.parse(t) for _, t, _ in specs]
     paulis = [[H.Pauli(n, x, z, ph) for x, z, ph in ps] for n, _, ps in specs]
@@ -187,26 +189,39 @@ def run_batch(args):
         for s in states.values():
             s.synchronize()
         t0 = time.perf_counter()
-        maxdiff = 0.0
+        evs = []
         for i in idx:
             s = states[specs[i][0]]
             s.set_basis(0)
             s.run(circs[i])
-            ev = s.expectations(paulis[i])
+            evs.append(s.expectations(paulis[i]))
+        per = (time.perf_counter() - t0) / len(idx)
+        maxdiff = 0.0
+        for i, ev in zip(idx, evs):
             bv = b.result(pos[i])[0]
             maxdiff = max(maxdiff, max(abs(a - c) for a, c in zip(ev, bv)))
-        per = (time.perf_counter() - t0) / len(idx)
+        diff128 = None
+        if prec == H.C64:  # each sampled circuit against its own complex128 run
+            states = {}
+            diff128 = 0.0
+            for i in idx:
+                s = states.setdefault(specs[i][0], H.HipStateVector.zero_state(specs[i][0], H.C128))
+                s.set_basis(0)
+                s.run(circs[i])
+                ev = s.expectations(paulis[i])
+                diff128 = max(diff128, max(abs(a - c) for a, c in zip(ev, b.result(pos[i])[0])))
+            del states
         log(f"per-circuit: {per * 1000:.2f} ms per circuit over {len(idx)} circuits; max |batch - single| = "
-            f"{maxdiff:.2e}")
-        return per, maxdiff
+            f"{maxdiff:.2e}" + ("" if diff128 is None else f"; max |batch - complex128 single| = {diff128:.2e}"))
+        return per, maxdiff, diff128
 
     allidx = list(range(len(specs)))
     b, wall, kms, cold = batched(allidx)
-    per, maxdiff = per_circuit(allidx[::args.stride], b, {i: i for i in allidx})
+    per, maxdiff, d128 = per_circuit(allidx[::args.stride], b, {i: i for i in allidx})
     del b
     small = [i for i in allidx if ns[i] <= 14]
     bs, wall_s, kms_s, _ = batched(small)
-    per_s, maxdiff_s = per_circuit(small[::max(1, args.stride // 2)], bs, {i: k for k, i in enumerate(small)})
+    per_s, maxdiff_s, d128_s = per_circuit(small[::max(1, args.stride // 2)], bs, {i: k for k, i in enumerate(small)})
     del bs
     amp_updates = float((gates * np.exp2(ns)).sum())
     cps = len(specs) / wall
@@ -221,6 +236,7 @@ def run_batch(args):
                            "ms_per_circuit": per * 1000,
                            "path": "single-state fused interpreter, one state per qubit count"},
            "speedup_vs_per_circuit": cps * per, "max_abs_batch_vs_single": max(maxdiff, maxdiff_s),
+           "max_abs_vs_complex128_single": None if d128 is None else max(d128, d128_s),
            "autotune": tuned,
            "small_circuits": {"qubits": "10-14", "count": len(small), "circuits_per_s": len(small) / wall_s,
                               "kernel_ms": kms_s, "per_circuit_circuits_per_s": 1.0 / per_s,
@@ -340,7 +356,8 @@ def main():
     ap.add_argument("--stride", type=int, default=8)
     ap.add_argument("--trajectories", type=int, default=2000)
     ap.add_argument("--single", type=int, default=3, help="trajectories timed through per-trajectory launches")
-    ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
+    ap.add_argument("--precision", default=None, choices=["c128", "c64"],
+                    help="default: complex128 for the brickwork, complex64 for the batch")
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
@@ -368,7 +385,7 @@ def main():
     paulis = [H.Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 64, 4)]
     circ = H.Circuit.parse(text)
     prec = H.C64 if args.precision == "c64" else H.C128
-    log(f"bench: n={n} depth={depth} gates={gates} paulis=64 fusion={args.fusion} precision={args.precision}")
+    log(f"bench: n={n} depth={depth} gates={gates} paulis=64 fusion={args.fusion} precision={args.precision or 'c128'}")
     s = H.HipStateVector.zero_state(n, prec)
     s.set_fusion(args.fusion)
 
