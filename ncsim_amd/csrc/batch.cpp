@@ -23,9 +23,9 @@ namespace ncsv {
 namespace batch {
 
 uint32_t max_tile_bits(ncsv_precision prec);
-int resident_blocks(ncsv_precision prec);
-void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipStream_t stream);
-void simulate(const BArgs &A, ncsv_precision prec);
+int resident_blocks(ncsv_precision prec, uint32_t threads);
+void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, uint32_t threads, int blocks, hipStream_t stream);
+void simulate(const BArgs &A, ncsv_precision prec, uint32_t threads);
 
 struct StepH {
     uint32_t L = 0;
@@ -937,6 +937,16 @@ uint32_t tile_bits_of(const ncsv_batch *b) {
     return mx;
 }
 
+// Threads per workgroup of the persistent kernel for tiles of Lb qubits:
+// 256 below the largest tile (half the LDS per workgroup, so twice as many
+// workgroups stay resident and one's loads overlap another's ops), else 512.
+// NCSV_BATCH_THREADS=512 keeps 512 for every tile width.
+uint32_t workgroup_threads(ncsv_precision prec, uint32_t Lb) {
+    const char *e = getenv("NCSV_BATCH_THREADS");
+    if (e && atoi(e) == 512) return 512;
+    return Lb < max_tile_bits(prec) ? 256u : 512u;
+}
+
 // Plans every entry's program (entries running the same circuit with the
 // same Paulis share one), in parallel host threads.
 void plan_all(ncsv_batch *b) {
@@ -1270,12 +1280,13 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
     A.next = (unsigned long long *)(w + o_next);
     A.flag = (uint32_t *)(w + o_flag);
     A.spin_limit = 1u << 22;  // about a second of waiting per item
-    int blocks = b->blocks > 0 ? b->blocks : resident_blocks(b->prec);
+    const uint32_t threads = workgroup_threads(b->prec, b->planned_bits);
+    int blocks = b->blocks > 0 ? b->blocks : resident_blocks(b->prec, threads);
     if ((uint64_t)blocks > ch.total) blocks = (int)std::max<uint64_t>(1, ch.total);
     // the same arguments as the captured launch: replay it
     std::vector<char> key(sizeof(BArgs) + 3 * sizeof(uint64_t));
     {
-        const uint64_t extra[3] = {(uint64_t)ch.recs.size(), (uint64_t)blocks, (uint64_t)b->prec};
+        const uint64_t extra[3] = {(uint64_t)ch.recs.size(), (uint64_t)blocks, (uint64_t)b->prec | (uint64_t)threads << 8};
         memcpy(key.data(), &A, sizeof(BArgs));
         memcpy(key.data() + sizeof(BArgs), extra, sizeof extra);
     }
@@ -1287,7 +1298,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
         b->graph = nullptr;
         check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
-            launch(A, (uint32_t)ch.recs.size(), b->prec, blocks, st);
+            launch(A, (uint32_t)ch.recs.size(), b->prec, threads, blocks, st);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(st, &g);
@@ -1304,7 +1315,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
     if (use_graph)
         check_hip(hipGraphLaunch(b->gexec, st), "hipGraphLaunch");
     else
-        launch(A, (uint32_t)ch.recs.size(), b->prec, blocks, st);
+        launch(A, (uint32_t)ch.recs.size(), b->prec, threads, blocks, st);
     check_hip(hipEventRecord(b->ev1, st), "hipEventRecord");
     ch.pexp.assign(ch.nexp + 1, 0.0);
     ch.outc.assign(ch.nmeas + 1, 0xff);
@@ -1355,7 +1366,7 @@ void run_chunk_sim(ncsv_batch *b, const Tables &t, Chunk &ch) {
     A.next = &next;
     A.flag = &flag;
     A.spin_limit = 1u << 20;
-    simulate(A, b->prec);
+    simulate(A, b->prec, workgroup_threads(b->prec, b->planned_bits));
     b->launches++;
     b->items += ch.total;
     if (flag & 1) fail(NCSV_DEVICE, "batched simulation: a step waited too long for its predecessor");

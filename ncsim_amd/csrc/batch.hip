@@ -19,7 +19,7 @@ template <> struct M4<double> { typedef double type __attribute__((ext_vector_ty
 }  // namespace ncsv
 
 #define NCSV_DEV __device__
-#define NCSV_KERNEL __global__ __launch_bounds__(BT, 4)
+#define NCSV_KERNEL __global__ __launch_bounds__(BT, 4)  // BT: the kernel's template parameter
 #define NCSV_SHARED __shared__
 #define NCSV_TID threadIdx.x
 #define NCSV_SYNC() __syncthreads()
@@ -54,7 +54,6 @@ __device__ inline uint64_t uniform64(uint64_t v) {
 
 namespace ncsv {
 namespace batch {
-using kern::BT;
 using kern::k_batch;
 namespace {
 
@@ -71,26 +70,38 @@ __global__ void k_batch_reset(uint32_t *done, uint32_t nrec, unsigned long long 
 
 uint32_t max_tile_bits(ncsv_precision prec) { return prec == NCSV_C128 ? 12u : 13u; }
 
-int resident_blocks(ncsv_precision prec) {
+int resident_blocks(ncsv_precision prec, uint32_t threads) {
     int dev = 0, cus = 0, per = 0;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-    if (prec == NCSV_C128)
-        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<double, 12>, BT, 0), "occupancy");
+    if (prec == NCSV_C128 && threads == 256)
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<double, 11, 256>, 256, 0), "occupancy");
+    else if (prec == NCSV_C128)
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<double, 12, 512>, 512, 0), "occupancy");
+    else if (threads == 256)
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<float, 12, 256>, 256, 0), "occupancy");
     else
-        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<float, 13>, BT, 0), "occupancy");
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<float, 13, 512>, 512, 0), "occupancy");
     return cus * std::max(per, 1);
 }
 
-void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, int blocks, hipStream_t stream) {
+// threads == 256: the workgroups of 256 threads, for tiles of at most
+// max_tile_bits - 1 qubits (half the LDS, twice the resident workgroups)
+void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, uint32_t threads, int blocks, hipStream_t stream) {
     hipLaunchKernelGGL(k_batch_reset, dim3((nrec + 255) / 256), dim3(256), 0, stream, A.done, nrec, A.next, A.flag);
     check_hip(hipGetLastError(), "k_batch_reset");
-    if (prec == NCSV_C128)
-        hipLaunchKernelGGL((k_batch<double, 12>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
-                           A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
+#define NCSV_LAUNCH(T, LB, BT)                                                                                        \
+    hipLaunchKernelGGL((k_batch<T, LB, BT>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis,     \
+                       A.mgroups, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A)
+    if (prec == NCSV_C128 && threads == 256)
+        NCSV_LAUNCH(double, 11, 256);
+    else if (prec == NCSV_C128)
+        NCSV_LAUNCH(double, 12, 512);
+    else if (threads == 256)
+        NCSV_LAUNCH(float, 12, 256);
     else
-        hipLaunchKernelGGL((k_batch<float, 13>), dim3(blocks), dim3(BT), 0, stream, A.steps, A.ops, A.coef, A.paulis, A.mgroups,
-                           A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A);
+        NCSV_LAUNCH(float, 13, 512);
+#undef NCSV_LAUNCH
     check_hip(hipGetLastError(), "k_batch");
 }
 

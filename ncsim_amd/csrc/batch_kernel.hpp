@@ -28,9 +28,6 @@ namespace ncsv {
 namespace batch {
 namespace kern {
 
-constexpr uint32_t BT = 512;        // threads per workgroup
-constexpr uint32_t NW = BT / 64;    // wavefronts per workgroup
-constexpr uint32_t LOB = 9;         // log2(BT): tile bits taken from the thread index
 constexpr uint32_t CMAX = 18;       // values per block reduction (9 Paulis)
 constexpr uint32_t SMAX = 4096;     // noise codes staged in LDS per record
 
@@ -93,8 +90,8 @@ template <typename V> NCSV_DEV inline uint32_t sw(uint32_t x) {
 
 // f(x, slot) for every x = deposit(j, mask, ones), j < NL >> popcount(mask).
 // deposit and the swizzle are linear over disjoint bits, so x splits into
-// the thread's part (j's low 9 bits) and a wave-uniform part per iteration.
-template <typename V, typename F>
+// the thread's part (j's low log2(BT) bits) and a wave-uniform part per iteration.
+template <typename V, uint32_t BT, typename F>
 NCSV_DEV inline void visit(uint32_t NL, uint32_t tid, uint32_t mask, uint32_t ones, F &&f) {
     const uint32_t cnt = NL >> __builtin_popcount(mask);
     const uint32_t xt = deposit(tid, mask, ones);
@@ -115,7 +112,8 @@ NCSV_DEV inline void visit(uint32_t NL, uint32_t tid, uint32_t mask, uint32_t on
 }
 
 // Sums v[0..cnt) over the workgroup (fixed order) into bc[0..cnt), which
-// stays valid until the next barrier-separated reduction.
+// stays valid until the next barrier-separated reduction (NW wavefronts).
+template <uint32_t NW>
 NCSV_DEV inline void block_sum(double (&v)[CMAX], uint32_t cnt, double *red, double *bc) {
     const uint32_t tid = NCSV_TID, lane = tid & 63, w = tid >> 6;
 #pragma unroll
@@ -153,7 +151,8 @@ NCSV_DEV inline uint32_t decide(const double *__restrict__ uni, const uint8_t *_
     return uni[ordinal] >= p_plus ? 1u : 0u;
 }
 
-template <typename T, uint32_t LB>
+// BT threads per workgroup (512 or 256); tiles of up to 2^LB amplitudes.
+template <typename T, uint32_t LB, uint32_t BT>
 NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restrict__ ops,
                          const double *__restrict__ cf, const BPauliD *__restrict__ paulis,
                          const BMGroupD *__restrict__ mgroups, const BEntryD *__restrict__ entries,
@@ -161,6 +160,9 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                          const double *__restrict__ uni, const uint8_t *__restrict__ forced,
                          const uint8_t *__restrict__ errs, BArgs A) {
     using V = typename V2<T>::type;
+    static_assert(BT == 512 || BT == 256, "workgroups of 512 or 256 threads");
+    constexpr uint32_t NW = BT / 64;                 // wavefronts per workgroup
+    constexpr uint32_t LOB = BT == 512 ? 9u : 8u;    // log2(BT): tile bits taken from the thread index
     NCSV_SHARED V tile[1u << LB];
     NCSV_SHARED double red[NW * CMAX];
     NCSV_SHARED double bc[CMAX];
@@ -292,7 +294,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     const V m0 = coef<V>(cf, op.coef), m1 = coef<V>(cf, op.coef + 1), m2 = coef<V>(cf, op.coef + 2),
                             m3 = coef<V>(cf, op.coef + 3);
                     const uint32_t sb = sw<V>(1u << op.p0);
-                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                    visit<V, BT>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;
                         const V a = tile[s0], b = tile[s1];
                         tile[s0] = cadd(cm(m0, a), cm(m1, b));
@@ -304,7 +306,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     const T r0 = (T)cf[2 * op.coef], r1 = (T)cf[2 * op.coef + 2], r2 = (T)cf[2 * op.coef + 4],
                             r3 = (T)cf[2 * op.coef + 6];
                     const uint32_t sb = sw<V>(1u << op.p0);
-                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                    visit<V, BT>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;
                         const V a = tile[s0], b = tile[s1];
                         V u, w;
@@ -322,7 +324,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     const T d0 = (T)cf[2 * op.coef], o1 = (T)cf[2 * op.coef + 3], o2 = (T)cf[2 * op.coef + 5],
                             d1 = (T)cf[2 * op.coef + 6];
                     const uint32_t sb = sw<V>(1u << op.p0);
-                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                    visit<V, BT>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;
                         const V a = tile[s0], b = tile[s1];
                         V u, w;
@@ -338,7 +340,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 case BK_U2: {
                     const uint32_t b0 = 1u << op.p0, b1 = 1u << op.p1;
                     const uint32_t c1 = sw<V>(b0), c2 = sw<V>(b1), c3 = c1 ^ c2;
-                    visit<V>(NL, tid, b0 | b1, 0, [&](uint32_t, uint32_t s) {
+                    visit<V, BT>(NL, tid, b0 | b1, 0, [&](uint32_t, uint32_t s) {
                         const uint32_t xs[4] = {s, s ^ c1, s ^ c2, s ^ c3};
                         V v[4];
 #pragma unroll
@@ -390,9 +392,9 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                         for (uint32_t a = 0; a < NA; a++) tile[xs[a]] = v[a];
                     };
                     if (K == 3)
-                        visit<V>(NL, tid, pb[0] | pb[1] | pb[2], 0, [&](uint32_t, uint32_t s) { body(s, KTag<3>{}); });
+                        visit<V, BT>(NL, tid, pb[0] | pb[1] | pb[2], 0, [&](uint32_t, uint32_t s) { body(s, KTag<3>{}); });
                     else
-                        visit<V>(NL, tid, pb[0] | pb[1], 0, [&](uint32_t, uint32_t s) { body(s, KTag<2>{}); });
+                        visit<V, BT>(NL, tid, pb[0] | pb[1], 0, [&](uint32_t, uint32_t s) { body(s, KTag<2>{}); });
                     break;
                 }
                 case BK_U3: {
@@ -486,14 +488,14 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 case BK_DM: {
                     if ((base & op.omask) != op.omask) continue;  // not this tile: no barrier
                     const V ph = coef<V>(cf, op.coef);
-                    visit<V>(NL, tid, op.lmask, op.lmask, [&](uint32_t, uint32_t s) { tile[s] = cm(ph, tile[s]); });
+                    visit<V, BT>(NL, tid, op.lmask, op.lmask, [&](uint32_t, uint32_t s) { tile[s] = cm(ph, tile[s]); });
                     break;
                 }
                 case BK_PERM1: {
                     if ((base & op.omask) != op.omask) continue;  // not this tile: no barrier
                     const uint32_t bp = 1u << op.p0, sb = sw<V>(bp);
                     const uint32_t k0 = op.k0, k1 = op.k1;
-                    visit<V>(NL, tid, op.lmask | bp, op.lmask, [&](uint32_t, uint32_t s0) {
+                    visit<V, BT>(NL, tid, op.lmask | bp, op.lmask, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;
                         const V a = tile[s0], b = tile[s1];
                         tile[s1] = ipow(k0, a);
@@ -503,7 +505,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 }
                 case BK_SWAP: {
                     const uint32_t b0 = 1u << op.p0, b1 = 1u << op.p1, sb = sw<V>(b0 | b1);
-                    visit<V>(NL, tid, b0 | b1, b0, [&](uint32_t, uint32_t s) {
+                    visit<V, BT>(NL, tid, b0 | b1, b0, [&](uint32_t, uint32_t s) {
                         const uint32_t s2 = s ^ sb;
                         const V a = tile[s], b = tile[s2];
                         tile[s] = b;
@@ -516,7 +518,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     const uint32_t code = NCSV_UNIFORM32(si < SMAX ? (uint32_t)scode[si] : (uint32_t)errs[en.err0 + op.a]);
                     if (!code) continue;  // no error at this site: the tile is unchanged, no barrier
                     const uint32_t sb = sw<V>(1u << op.p0);
-                    visit<V>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
+                    visit<V, BT>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;
                         V a = tile[s0], b = tile[s1];
                         if (code & 2) {
@@ -561,7 +563,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                             for (uint32_t c = 0; c < 16; c++) v[c] += c < NC ? x[c] : 0.0;
                         }
                     } else {
-                        visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                        visit<V, BT>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
                             const V a = tile[s];
                             const double w = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
                             const uint32_t cx = cell(x);
@@ -569,7 +571,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                             for (uint32_t c = 0; c < 16; c++) v[c] += (c == cx) ? w : 0.0;
                         });
                     }
-                    block_sum(v, NC, red, bc);
+                    block_sum<NW>(v, NC, red, bc);
                     if (op.kind == BK_REDUCE) {
                         if (tid < NC) A.part[en.part0 + op.a + (uint64_t)NC * tileno + tid] = bc[tid];
                         break;
@@ -606,7 +608,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     }
                     if (tid == 0 && tileno == 0 && lfac != 1.0) A.lsc[rec.entry] *= lfac;
                     const T sc = (T)scale;
-                    visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                    visit<V, BT>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
                         V a = tile[s];
                         if (!((allow >> cell(x)) & 1)) {
                             a.x = (T)0;
@@ -621,7 +623,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                         // resets with outcome 1: x -> x ^ flip
                         NCSV_SYNC();
                         const uint32_t sf = sw<V>(flip);
-                        visit<V>(NL, tid, flip & (0u - flip), 0, [&](uint32_t, uint32_t s) {
+                        visit<V, BT>(NL, tid, flip & (0u - flip), 0, [&](uint32_t, uint32_t s) {
                             const uint32_t s2 = s ^ sf;
                             const V a = tile[s], b = tile[s2];
                             tile[s] = b;
@@ -635,7 +637,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     double v[CMAX];
 #pragma unroll
                     for (uint32_t k = 0; k < CMAX; k++) v[k] = 0.0;
-                    visit<V>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
+                    visit<V, BT>(NL, tid, 0, 0, [&](uint32_t x, uint32_t s) {
                         const V a = tile[s];
                         const double ax = (double)a.x, ay = (double)a.y;
 #pragma unroll
@@ -663,7 +665,7 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                             }
                         }
                     });
-                    block_sum(v, 2 * K, red, bc);
+                    block_sum<NW>(v, 2 * K, red, bc);
                     if (tid < 2 * K) A.pexp[en.exp0 + op.a + (uint64_t)2 * K * tileno + tid] = bc[tid];
                     break;
                 }

@@ -1,5 +1,5 @@
 // Host build of the batched form's kernel body (batch_kernel.hpp): one
-// workgroup runs as BT host threads, with a real barrier for each
+// workgroup runs as 512 or 256 host threads, with a real barrier for each
 // workgroup barrier and a shared exchange for the wavefront shuffle, so the
 // kernel's own logic (queue, dependencies, ops, reductions) can be run and
 // checked without a device.  One workgroup claims every item in order, so
@@ -114,23 +114,29 @@ double shfl_xor(double x, unsigned mask) {
 namespace ncsv {
 namespace batch {
 
-// Runs the kernel body over the whole queue with one simulated workgroup.
-void simulate(const BArgs &A, ncsv_precision prec) {
+// Runs the kernel body over the whole queue with one simulated workgroup of
+// `threads` threads (512, or 256 for tiles below the largest).
+void simulate(const BArgs &A, ncsv_precision prec, uint32_t threads) {
     static std::mutex one;  // the simulated workgroup's LDS is static
     std::lock_guard<std::mutex> lk(one);
-    sim::Barrier b(kern::BT);
+    sim::Barrier b(threads);
     sim::bar = &b;
     std::vector<std::thread> th;
-    th.reserve(kern::BT);
-    for (unsigned t = 0; t < kern::BT; t++) {
-        th.emplace_back([&A, prec, t]() {
+    th.reserve(threads);
+    for (unsigned t = 0; t < threads; t++) {
+        th.emplace_back([&A, prec, t, threads]() {
             sim::tid = t;
-            if (prec == NCSV_C128)
-                kern::k_batch<double, 12>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.entries, A.recs, A.rec_blk,
-                                          A.uni, A.forced, A.errs, A);
+#define NCSV_RUN(T, LB, BT) \
+    kern::k_batch<T, LB, BT>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.entries, A.recs, A.rec_blk, A.uni, A.forced, A.errs, A)
+            if (prec == NCSV_C128 && threads == 256)
+                NCSV_RUN(double, 11, 256);
+            else if (prec == NCSV_C128)
+                NCSV_RUN(double, 12, 512);
+            else if (threads == 256)
+                NCSV_RUN(float, 12, 256);
             else
-                kern::k_batch<float, 13>(A.steps, A.ops, A.coef, A.paulis, A.mgroups, A.entries, A.recs, A.rec_blk,
-                                         A.uni, A.forced, A.errs, A);
+                NCSV_RUN(float, 13, 512);
+#undef NCSV_RUN
         });
     }
     for (auto &x : th) x.join();

@@ -293,15 +293,22 @@ def test_dense_blocks_fuse_brickwork(capfd, monkeypatch):
     assert any(line.strip().startswith("U3") and not line.endswith("o=0") for line in fused)
 
 
-@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
-def test_kernel_code_on_host_matches_oracle(precision):
+@pytest.mark.parametrize("precision,threads,tile_bits", [
+    (nc.C128, "512", 7), (nc.C64, "512", 7), (nc.C128, "256", 7), (nc.C64, "256", 7),
+    (nc.C128, "256", 11), (nc.C64, "256", 12), (nc.C128, "512", 12)])
+def test_kernel_code_on_host_matches_oracle(precision, threads, tile_bits, monkeypatch):
     """The device kernel's own source run on the host (one workgroup as 512
-    threads with real barriers): whole-state entries and a multi-tile entry
-    whose steps wait on each other."""
+    or 256 threads with real barriers): whole-state entries and multi-tile
+    entries whose steps wait on each other, on tiles below the thread bits
+    and on the widest tile each workgroup size runs (chunked tile loads,
+    paired MFMA rounds)."""
+    monkeypatch.setenv("NCSV_BATCH_THREADS", threads)
     cases = [(5, 4, 31), (8, 5, 32), (9, 4, 33)]
+    if tile_bits > 7:
+        cases += [(tile_bits + 1, 4, 34), (tile_bits, 3, 35)]
     texts = [random_circuit(n, d, s) for n, d, s in cases]
     paulis = [random_paulis(n, 3, s) for n, d, s in cases]
-    b = _build(texts, paulis, precision, 7)
+    b = _build(texts, paulis, precision, tile_bits)
     us = _draws(b, 21)
     b.simulate(uniforms=us)
     tol = 1e-12 if precision == nc.C128 else 2e-5
