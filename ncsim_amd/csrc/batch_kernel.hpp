@@ -287,8 +287,14 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
             }
             NCSV_SYNC();
 
+            // a program's first step starts from |0...0>: tiles other than the
+            // first hold zeros throughout it (no op moves amplitudes between
+            // tiles), so their gate ops are skipped; reductions still run and
+            // write their zero partials
+            const bool zero_tile = (st.flags & BF_INIT) && tileno != 0;
             for (uint32_t oi = op0; oi < op1; oi++) {
                 const BOpD op = ops[oi];
+                if (zero_tile && (op.kind <= BK_NSLOT || op.kind >= BK_G3)) continue;  // the tile is unchanged: no barrier
                 switch (op.kind) {
                 case BK_U1: {
                     const V m0 = coef<V>(cf, op.coef), m1 = coef<V>(cf, op.coef + 1), m2 = coef<V>(cf, op.coef + 2),
