@@ -33,6 +33,7 @@ struct StepH {
     std::vector<BOpD> ops;    // as planned
     std::vector<BOpD> gops, gopsc;  // grouped; grouped without noise slots
     std::vector<uint32_t> slots;    // noise slots of this step
+    std::vector<std::vector<BOpD>> gops1;  // grouped keeping only slot k (empty: none built)
     std::vector<cd> coef;
     std::vector<BPauliD> paulis;
     std::vector<BMGroupD> mgroups;
@@ -821,15 +822,26 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     const bool fuse = block_fusion_enabled();
     const bool f32 = prec == NCSV_C64;
     for (StepH &st : P->steps) {
-        if (fuse) {
+        auto plan = [&](const std::vector<BOpD> &ops, bool drop_slots) {
+            if (!fuse) return group_ops(&st, ops, drop_slots);
+            if (!drop_slots) return group_ops(&st, fuse_blocks(&st, ops, f32), false);
             std::vector<BOpD> clean;
-            for (const BOpD &o : st.ops)
+            for (const BOpD &o : ops)
                 if (o.kind != BK_NSLOT) clean.push_back(o);
-            st.gops = group_ops(&st, fuse_blocks(&st, st.ops, f32), false);
-            st.gopsc = group_ops(&st, fuse_blocks(&st, clean, f32), true);
-        } else {
-            st.gops = group_ops(&st, st.ops, false);
-            st.gopsc = group_ops(&st, st.ops, true);
+            return group_ops(&st, fuse_blocks(&st, clean, f32), true);
+        };
+        st.gops = plan(st.ops, false);
+        st.gopsc = plan(st.ops, true);
+        // one list per noise slot keeping only that slot: a step where one
+        // error was drawn runs it, with the other slots' sites fused over
+        if (st.slots.size() >= 2 && st.slots.size() <= 256) {
+            st.gops1.resize(st.slots.size());
+            for (size_t k = 0; k < st.slots.size(); k++) {
+                std::vector<BOpD> one;
+                for (const BOpD &o : st.ops)
+                    if (o.kind != BK_NSLOT || o.a == st.slots[k]) one.push_back(o);
+                st.gops1[k] = plan(one, false);
+            }
         }
     }
     if (getenv("NCSV_BATCH_DUMP")) {
@@ -1009,6 +1021,19 @@ template <typename T> T *at(void *base, size_t off) { return (T *)((char *)base 
 size_t up256(size_t v) { return (v + 255) / 256 * 256; }
 
 // The programs' steps, ops, coefficients, Paulis and qubit tables, uploaded once.
+// The slot (index in the step's slot list) of the one error drawn in this
+// step when the step has single-slot lists and exactly one error; else -1.
+int single_error(const StepH &s, const uint8_t *errs) {
+    if (s.gops1.empty()) return -1;
+    int k = -1;
+    for (size_t i = 0; i < s.slots.size(); i++)
+        if (errs[s.slots[i]]) {
+            if (k >= 0) return -1;
+            k = (int)i;
+        }
+    return k;
+}
+
 struct Tables {
     std::vector<BStepD> steps;
     std::vector<BOpD> ops;
@@ -1054,6 +1079,18 @@ Tables flatten(ncsv_batch *b) {
             d.op0c = (uint32_t)t.ops.size();
             d.nopsc = (uint32_t)s.gopsc.size();
             put(s.gopsc);
+            if (!s.gops1.empty()) {
+                // headers first (never run as ops), then the lists
+                d.var0 = (uint32_t)t.ops.size();
+                t.ops.resize(t.ops.size() + s.gops1.size(), BOpD{});
+                for (size_t k = 0; k < s.gops1.size(); k++) {
+                    BOpD &h = t.ops[d.var0 + k];
+                    h.kind = 0xff;
+                    h.a = (uint32_t)t.ops.size();
+                    h.b = (uint32_t)s.gops1[k].size();
+                    put(s.gops1[k]);
+                }
+            }
             for (const cd &c : s.coef) {
                 t.coef.push_back(c.real());
                 t.coef.push_back(c.imag());
@@ -1202,6 +1239,11 @@ Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *
             uint32_t err = 0;
             for (uint32_t sl : P.steps[d].slots) err |= ch.errs[ch.ed[i].err0 + sl];
             r.flags = (err ? RF_ERR : 0u) | (d + 1 < P.steps.size() ? RF_RELEASE : 0u);
+            const int one = single_error(P.steps[d], &ch.errs[ch.ed[i].err0]);
+            if (one >= 0) {
+                r.flags |= RF_ONE;
+                r.var = (uint32_t)one;
+            }
             // at least 8 items per step where the step has that many tiles
             const uint32_t lg = P.n - P.steps[d].L;
             r.lgt = std::min(lg > 3 ? lg - 3 : 0u, lg_tiles_per_item);
@@ -1420,7 +1462,12 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                     tl[x] = (st.flags & BF_INIT) ? cd((x == 0 && tileno == 0) ? 1.0 : 0.0, 0.0) : amp[addr[x]];
                 uint32_t err = 0;
                 for (uint32_t sl : P.steps[si].slots) err |= ch.errs[en.err0 + sl];
-                const uint32_t op0 = err ? st.op0 : st.op0c, op1 = op0 + (err ? st.nops : st.nopsc);
+                uint32_t op0 = err ? st.op0 : st.op0c, op1 = op0 + (err ? st.nops : st.nopsc);
+                const int one = single_error(P.steps[si], &ch.errs[en.err0]);
+                if (one >= 0) {
+                    op0 = t.ops[st.var0 + (uint32_t)one].a;
+                    op1 = op0 + t.ops[st.var0 + (uint32_t)one].b;
+                }
                 for (uint32_t oi = op0; oi < op1; oi++) {
                     const BOpD &op = t.ops[oi];
                     auto C = [&](uint32_t k) { return cd(t.coef[2 * (op.coef + k)], t.coef[2 * (op.coef + k) + 1]); };

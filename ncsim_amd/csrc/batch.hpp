@@ -86,6 +86,8 @@ struct BStepD {
     uint32_t qoff;   // qubit table: tile qubit of positions 0..L-1, then outer qubits ascending
     uint32_t omask;  // the outer qubits (tile number bit b <-> the b-th lowest)
     uint32_t slot_lo, slot_n;  // the noise slots of this step lie in [slot_lo, slot_lo + slot_n)
+    uint32_t var0;             // ops[var0 + k]: header (a = first op, b = op count) of the op list
+                               // keeping only the step's k-th noise slot (records with one error)
     uint8_t tpos[16];          // the qubits of tile positions 0..min(L, 16)-1
 };
 
@@ -104,9 +106,12 @@ struct BRecD {
     uint32_t prev_tiles;  // its tile count
     uint32_t flags;       // RF_ERR: a noise error was drawn in this step; RF_RELEASE: a later step waits on it
     uint32_t lgt;         // log2 of the tiles per item
+    uint32_t var;         // RF_ONE: the step's noise slot (index in its slot list) holding the one error
 };
 
-enum BRecFlags : uint32_t { RF_ERR = 1, RF_RELEASE = 2 };
+// RF_ONE (with RF_ERR): exactly one error was drawn in this step, so the op
+// list keeping only its slot runs instead of the list with every slot
+enum BRecFlags : uint32_t { RF_ERR = 1, RF_RELEASE = 2, RF_ONE = 4 };
 
 struct BArgs {
     const BStepD *steps;

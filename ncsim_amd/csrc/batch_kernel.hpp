@@ -233,8 +233,14 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 if ((k >> b) & 1) h |= wq[b];
             return h;
         };
-        const uint32_t op0 = (rec.flags & RF_ERR) ? st.op0 : st.op0c;
-        const uint32_t op1 = op0 + ((rec.flags & RF_ERR) ? st.nops : st.nopsc);
+        uint32_t op0 = (rec.flags & RF_ERR) ? st.op0 : st.op0c;
+        uint32_t op1 = op0 + ((rec.flags & RF_ERR) ? st.nops : st.nopsc);
+        if (rec.flags & RF_ONE) {
+            // one error drawn in this step: the list keeping only its slot
+            const BOpD h = ops[st.var0 + rec.var];
+            op0 = h.a;
+            op1 = h.a + h.b;
+        }
         if (rec.flags & RF_ERR) {
             // this entry's noise codes for the step's slots (read after the first
             // tile's load barrier; the previous record ended with a barrier)
