@@ -822,13 +822,32 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     const bool fuse = block_fusion_enabled();
     const bool f32 = prec == NCSV_C64;
     for (StepH &st : P->steps) {
-        auto plan = [&](const std::vector<BOpD> &ops, bool drop_slots) {
-            if (!fuse) return group_ops(&st, ops, drop_slots);
-            if (!drop_slots) return group_ops(&st, fuse_blocks(&st, ops, f32), false);
-            std::vector<BOpD> clean;
-            for (const BOpD &o : ops)
-                if (o.kind != BK_NSLOT) clean.push_back(o);
-            return group_ops(&st, fuse_blocks(&st, clean, f32), true);
+        auto plan = [&](const std::vector<BOpD> &all, bool drop_slots) {
+            // phases on outer qubits only multiply whole tiles and commute with
+            // every op of the step: they leave the op passes and are taken at
+            // the store, as one phase per tile
+            std::vector<BOpD> ops, tph;
+            for (BOpD o : all) {
+                if (o.kind == BK_DM && !o.lmask && o.omask) {
+                    o.kind = BK_OPHASE;
+                    tph.push_back(o);
+                } else {
+                    ops.push_back(o);
+                }
+            }
+            std::vector<BOpD> out;
+            if (!fuse) {
+                out = group_ops(&st, ops, drop_slots);
+            } else if (!drop_slots) {
+                out = group_ops(&st, fuse_blocks(&st, ops, f32), false);
+            } else {
+                std::vector<BOpD> clean;
+                for (const BOpD &o : ops)
+                    if (o.kind != BK_NSLOT) clean.push_back(o);
+                out = group_ops(&st, fuse_blocks(&st, clean, f32), true);
+            }
+            out.insert(out.end(), tph.begin(), tph.end());
+            return out;
         };
         st.gops = plan(st.ops, false);
         st.gopsc = plan(st.ops, true);
@@ -846,14 +865,14 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     }
     if (getenv("NCSV_BATCH_DUMP")) {
         static const char *nm[] = {"U1", "U1R", "U1X", "U2", "DM", "PERM1", "SWAP", "NSLOT",
-                                   "MEAS", "RED", "COLL", "EXP", "G3", "U3"};
+                                   "MEAS", "RED", "COLL", "EXP", "G3", "U3", "OPH"};
         for (size_t i = 0; i < P->steps.size(); i++) {
             const StepH &st = P->steps[i];
-            int cnt[14] = {0};
+            int cnt[15] = {0};
             for (const BOpD &o : st.gopsc) cnt[o.kind]++;
             fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu:", i, st.L, st.flags, st.gops.size(),
                     st.gopsc.size(), st.slots.size());
-            for (int k = 0; k < 14; k++)
+            for (int k = 0; k < 15; k++)
                 if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
             fprintf(stderr, "\n");
             if (atoi(getenv("NCSV_BATCH_DUMP")) >= 2) {
@@ -1066,7 +1085,7 @@ Tables flatten(ncsv_batch *b) {
                            g0 = (uint32_t)t.mgroups.size();
             auto put = [&](const std::vector<BOpD> &ops) {
                 for (BOpD o : ops) {
-                    if (o.kind <= BK_DM || o.kind == BK_G3 || o.kind == BK_U3) o.coef += c0;
+                    if (o.kind <= BK_DM || o.kind == BK_G3 || o.kind == BK_U3 || o.kind == BK_OPHASE) o.coef += c0;
                     if (o.kind == BK_U3) o.a += c0;
                     if (o.kind == BK_EXPECT) o.c += p0;
                     if (o.kind == BK_MEASURE || o.kind == BK_REDUCE || o.kind == BK_COLLAPSE) o.c += g0;
@@ -1528,6 +1547,7 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                         }
                         break;
                     case BK_DM:
+                    case BK_OPHASE:  // lmask 0: the whole tile
                         if ((base & op.omask) != op.omask) break;
                         for (uint32_t x = 0; x < NL; x++)
                             if ((x & op.lmask) == op.lmask) tl[x] *= C(0);

@@ -45,6 +45,8 @@ enum BOpKind : uint8_t {
     BK_G3 = 12,     // k1 (2 or 3) unconditional 2x2s on distinct positions p0, p1, p2 in one pass
     BK_U3 = 13,     // dense 8x8 on distinct positions p0, p1, p2 (coef: 64, row-major, then 64 more for tiles
                     // meeting omask when omask != 0; a: the FP32 copies in complex64)
+    BK_OPHASE = 14, // phase coef[0] on the whole tile where the outer bits omask are set, multiplied in at
+                    // the store (a phase on outer qubits only commutes with every op of the step)
 };
 
 struct BOpD {

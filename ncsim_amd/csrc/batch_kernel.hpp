@@ -298,9 +298,21 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
             // tiles), so their gate ops are skipped; reductions still run and
             // write their zero partials
             const bool zero_tile = (st.flags & BF_INIT) && tileno != 0;
+            // the product of the step's phases on outer qubits met by this tile,
+            // multiplied into the amplitudes at the store
+            double tpr = 1.0, tpi = 0.0;
             for (uint32_t oi = op0; oi < op1; oi++) {
                 const BOpD op = ops[oi];
                 if (zero_tile && (op.kind <= BK_NSLOT || op.kind >= BK_G3)) continue;  // the tile is unchanged: no barrier
+                if (op.kind == BK_OPHASE) {
+                    if ((base & op.omask) == op.omask) {
+                        const double cr = cf[2 * op.coef], ci = cf[2 * op.coef + 1];
+                        const double r = tpr * cr - tpi * ci;
+                        tpi = tpr * ci + tpi * cr;
+                        tpr = r;
+                    }
+                    continue;  // no pass: no barrier
+                }
                 switch (op.kind) {
                 case BK_U1: {
                     const V m0 = coef<V>(cf, op.coef), m1 = coef<V>(cf, op.coef + 1), m2 = coef<V>(cf, op.coef + 2),
@@ -690,17 +702,28 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 V *gs = g + base;
                 uint32_t los = offlo;
                 NCSV_OPAQUE(los);  // recomputed here, not held live across the ops
-                if (kcnt >= CH) {
-                    for (uint32_t k0 = 0; k0 < kcnt; k0 += CH) {
-                        const uint32_t h0 = los | hibits(k0);
+                V pv;
+                pv.x = (T)tpr;
+                pv.y = (T)tpi;
+                auto store = [&](auto tag) {
+                    constexpr bool PH = decltype(tag)::value != 0;  // multiply by the tile's phase
+                    auto val = [&](uint32_t s) { return PH ? cm(pv, tile[s]) : tile[s]; };
+                    if (kcnt >= CH) {
+                        for (uint32_t k0 = 0; k0 < kcnt; k0 += CH) {
+                            const uint32_t h0 = los | hibits(k0);
 #pragma unroll
-                        for (uint32_t c = 0; c < CH; c++) gs[h0 | hibits(c)] = tile[st_tid ^ sw<V>(BT * (k0 + c))];
+                            for (uint32_t c = 0; c < CH; c++) gs[h0 | hibits(c)] = val(st_tid ^ sw<V>(BT * (k0 + c)));
+                        }
+                    } else if (full) {
+                        for (uint32_t k = 0; k < kcnt; k++) gs[los | hibits(k)] = val(st_tid ^ sw<V>(BT * k));
+                    } else if (tid < NL) {
+                        gs[los] = val(st_tid);
                     }
-                } else if (full) {
-                    for (uint32_t k = 0; k < kcnt; k++) gs[los | hibits(k)] = tile[st_tid ^ sw<V>(BT * k)];
-                } else if (tid < NL) {
-                    gs[los] = tile[st_tid];
-                }
+                };
+                if (tpr != 1.0 || tpi != 0.0)
+                    store(KTag<1>{});
+                else
+                    store(KTag<0>{});
             }
         }  // tiles of the item
         if (rec.flags & RF_RELEASE) {

@@ -293,6 +293,63 @@ def test_dense_blocks_fuse_brickwork(capfd, monkeypatch):
     assert any(line.strip().startswith("U3") and not line.endswith("o=0") for line in fused)
 
 
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+def test_outer_phases_taken_at_the_store(precision, capfd, monkeypatch):
+    """Phases on qubits outside the tile (here the next layer's first RZ on
+    the outer qubits, after a measure-and-reset) become one phase per tile
+    multiplied in at the store, without a pass of their own; a noisy 9-qubit
+    brickwork on 6-qubit tiles, with errors drawn (p = 0.05), equals the
+    oracle through the FP64 emulation and through the kernel's source on the
+    host."""
+    from ncsim_amd import workloads as W
+    monkeypatch.setenv("NCSV_BATCH_DUMP", "2")
+    text = W.noisy_brickwork(9, 8, 3, p=0.05, every=4)
+    paulis = random_paulis(9, 6, 51)
+    b = _build([text], [paulis], precision, 6)
+    capfd.readouterr()
+    b.plan()
+    step, later = -1, 0
+    for line in capfd.readouterr().err.splitlines():
+        if line.startswith("step"):
+            step = int(line.split()[1])
+        elif line.strip().startswith("OPH") and step > 0:
+            later += 1
+    assert later >= 3
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    for seed in (52, 53):
+        us = _draws(b, seed)
+        rex, rnn, roc = oracle_entry(text, paulis, us[0])
+        # the kernel's own source once (complex64), the emulation for every case
+        runs = (b.emulate, b.simulate) if precision == nc.C64 and seed == 52 else (b.emulate,)
+        for run in runs:
+            run(uniforms=us)
+            ex, nn, oc = b.result(0)
+            assert oc == roc
+            assert abs(nn - rnn) < tol
+            assert max(abs(a - r) for a, r in zip(ex, rex)) < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+def test_gpu_outer_phases_taken_at_the_store(precision):
+    """The per-tile phases of outer-only diagonals on the device: noisy
+    brickworks on tiles smaller than the state against the oracle."""
+    from ncsim_amd import workloads as W
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    for n, L in ((9, 6), (12, 8), (14, 10)):
+        text = W.noisy_brickwork(n, 8, 3, p=0.05, every=4)
+        paulis = random_paulis(n, 6, 51)
+        b = _build([text], [paulis], precision, L)
+        for seed in (52, 53, 54):
+            us = _draws(b, seed)
+            b.run(uniforms=us)
+            ex, nn, oc = b.result(0)
+            rex, rnn, roc = oracle_entry(text, paulis, us[0])
+            assert oc == roc, (n, L, seed)
+            assert abs(nn - rnn) < tol
+            assert max(abs(a - r) for a, r in zip(ex, rex)) < tol, (n, L, seed)
+
+
 @pytest.mark.parametrize("precision,threads,tile_bits", [
     (nc.C128, "512", 7), (nc.C64, "512", 7), (nc.C128, "256", 7), (nc.C64, "256", 7),
     (nc.C128, "256", 11), (nc.C64, "256", 12), (nc.C128, "512", 12)])
