@@ -4,7 +4,9 @@ The batched kernel's speed depends on three planning choices that trade
 against each other differently per workload and device:
 
 - tile width (qubits per fused sweep, at most 13 for complex64 and 12 for
-  complex128): wider tiles cover more gates per pass over the state;
+  complex128): wider tiles cover more gates per pass over the state, and
+  tiles below the largest run in workgroups of 256 threads (four resident
+  per CU instead of two);
 - fixed low tile qubits (5 / 4 by default): more give longer contiguous
   runs per load, fewer leave the planner more freedom (fewer sweeps);
 - tiles per queue item: longer items need fewer device-scope releases,
@@ -12,7 +14,7 @@ against each other differently per workload and device:
 
 `tune_batch` times each candidate on a stratified sample of the batch's own
 entries (same circuits, same Paulis, seeded draws), applies the fastest to
-the batch and records it in a JSON cache keyed by device, precision and the
+the batch (near-ties to the larger queue item) and records it in a JSON cache keyed by device, precision and the
 batch's shape (qubit-count histogram, draws or none), so later batches of
 the same kind skip the search.
 """
@@ -63,6 +65,19 @@ def _apply(batch, cfg):
     batch.set_shape(cfg["low_bits"], cfg["tiles_per_item"])
 
 
+def choose(timings):
+    """The candidate to apply from [(config, kernel_ms), ...]: the fastest,
+    with near-ties (within 3 %, about the samples' run-to-run spread) going
+    to the larger queue item.  A sample holds a small share of the full
+    batch's items, and the device-scope release at each item end weighs
+    more at full size (2000 noisy 24-qubit trajectories: 12.4 s with 128
+    tiles per item against 13.2 s with 32, within 2 % of each other on a
+    48-trajectory sample)."""
+    fastest = min(t[1] for t in timings)
+    near = [t for t in timings if t[1] <= fastest * 1.03]
+    return max(near, key=lambda t: (t[0]["tiles_per_item"], -t[1]))[0]
+
+
 def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None):
     """Chooses and applies the fastest candidate; returns
     {"config": ..., "timings": [(config, kernel_ms), ...], "cached": bool}."""
@@ -102,7 +117,7 @@ def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None):
         if log:
             log(f"autotune {cfg}: {best:.2f} ms")
         del b
-    cfg = min(timings, key=lambda t: t[1])[0]
+    cfg = choose(timings)
     _apply(batch, cfg)
     db[key] = {"config": cfg, "timings": timings, "sample": len(idx), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
     try:

@@ -215,6 +215,7 @@ bool as_2x2(const StepH &s, const BOpD &o, uint32_t *pos, cd m[4]) {
     case BK_U1:
     case BK_U1R:
     case BK_U1X:
+        if (o.omask) return false;  // two 2x2s chosen by the outer condition
         *pos = o.p0;
         for (int t = 0; t < 4; t++) m[t] = s.coef[o.coef + t];
         return true;
@@ -380,6 +381,29 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in, bool fp32_c
         prev.dense = dense;
         prev.start = out.size();
         prev.b = b;
+        if (!dense && __builtin_popcount(b.mask) == 1 && b.omask && b.ops.size() >= 2) {
+            // ops on one position, some conditioned on outer qubits: one 2x2
+            // per outcome of the condition, in one pass
+            const uint32_t p = (uint32_t)__builtin_ctz(b.mask);
+            cd M[8];
+            for (uint32_t var = 0; var < 2; var++)
+                for (uint32_t c = 0; c < 2; c++) {
+                    cd v[2] = {};
+                    v[c] = 1;
+                    for (const BOpD &o : b.ops) apply_local(*s, o, &p, 1, v, var == 1);
+                    M[4 * var + c] = v[0];
+                    M[4 * var + 2 + c] = v[1];
+                }
+            BOpD d{};
+            d.kind = BK_U1;
+            d.p0 = (uint8_t)p;
+            d.omask = b.omask;
+            d.k0 = M[0] == cd(1, 0) && M[1] == cd(0, 0) && M[2] == cd(0, 0) && M[3] == cd(1, 0);
+            d.coef = (uint32_t)s->coef.size();
+            s->coef.insert(s->coef.end(), M, M + 8);
+            out.push_back(d);
+            return;
+        }
         if (!dense) {
             out.insert(out.end(), b.ops.begin(), b.ops.end());
             return;
@@ -1524,14 +1548,18 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                     }
                     case BK_U1:
                     case BK_U1R:
-                    case BK_U1X:
+                    case BK_U1X: {
+                        // a U1 with an outer condition: the second 2x2 where it holds
+                        const uint32_t vo = op.omask && (base & op.omask) == op.omask ? 4u : 0u;
+                        if (op.omask && !vo && op.k0) break;
                         for (uint32_t j = 0; j < NL / 2; j++) {
                             const uint32_t x0 = ins0h(j, op.p0), x1 = x0 | (1u << op.p0);
                             const cd a = tl[x0], v = tl[x1];
-                            tl[x0] = C(0) * a + C(1) * v;
-                            tl[x1] = C(2) * a + C(3) * v;
+                            tl[x0] = C(vo) * a + C(vo + 1) * v;
+                            tl[x1] = C(vo + 2) * a + C(vo + 3) * v;
                         }
                         break;
+                    }
                     case BK_U2:
                         for (uint32_t j = 0; j < NL / 4; j++) {
                             const uint32_t x = ins0h(ins0h(j, op.p0), op.p1);

@@ -30,7 +30,8 @@ namespace ncsv {
 namespace batch {
 
 enum BOpKind : uint8_t {
-    BK_U1 = 0,      // 2x2 on position p0 (coef: 4)
+    BK_U1 = 0,      // 2x2 on position p0 (coef: 4; when omask != 0, 4 more for tiles meeting omask, and
+                    // k0 = 1: tiles that do not meet it are left as they are)
     BK_U1R = 1,     // 2x2 with real entries
     BK_U1X = 2,     // real diagonal, imaginary off-diagonal
     BK_U2 = 3,      // 4x4 on positions p0 < p1 (coef: 16)

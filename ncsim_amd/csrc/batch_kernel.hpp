@@ -315,8 +315,16 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                 }
                 switch (op.kind) {
                 case BK_U1: {
-                    const V m0 = coef<V>(cf, op.coef), m1 = coef<V>(cf, op.coef + 1), m2 = coef<V>(cf, op.coef + 2),
-                            m3 = coef<V>(cf, op.coef + 3);
+                    // with an outer condition: the second 2x2 for tiles meeting it; k0 = 1
+                    // leaves the other tiles as they are
+                    uint32_t c0 = op.coef;
+                    if (op.omask) {
+                        if ((base & op.omask) == op.omask)
+                            c0 += 4;
+                        else if (op.k0)
+                            continue;  // not this tile: no barrier
+                    }
+                    const V m0 = coef<V>(cf, c0), m1 = coef<V>(cf, c0 + 1), m2 = coef<V>(cf, c0 + 2), m3 = coef<V>(cf, c0 + 3);
                     const uint32_t sb = sw<V>(1u << op.p0);
                     visit<V, BT>(NL, tid, 1u << op.p0, 0, [&](uint32_t, uint32_t s0) {
                         const uint32_t s1 = s0 ^ sb;

@@ -350,6 +350,56 @@ def test_gpu_outer_phases_taken_at_the_store(precision):
             assert max(abs(a - r) for a, r in zip(ex, rex)) < tol, (n, L, seed)
 
 
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+def test_conditioned_one_position_runs(precision, capfd, monkeypatch):
+    """Ops on one tile position under one condition on outer qubits (a CX
+    whose control is outside the tile, then the target's next gate) become
+    one 2x2 per outcome of the condition, in one pass: a 12-qubit circuit of
+    the 10-22 qubit mix on 6-qubit tiles plans five of them, and the program
+    equals the oracle through the FP64 emulation (the kernel runs them in
+    test_gpu_conditioned_one_position_runs)."""
+    from ncsim_amd import workloads as W
+    monkeypatch.setenv("NCSV_BATCH_DUMP", "2")
+    n, text, _ = W.batch_circuits(64, 11)[35]
+    paulis = [nc.Pauli(n, 1 << q, z, 0) for q, z in ((0, 0), (3, 8), (5, 0), (7, 1), (9, 0), (11, 4))]
+    paulis += [nc.Pauli(n, (1 << q) | (1 << (q + 1)), 1 << q, 1) for q in (1, 4, 6, 8, 10)]
+    b = _build([text], [paulis], precision, 6)
+    capfd.readouterr()
+    b.plan()
+    cond = [line for line in capfd.readouterr().err.splitlines()
+            if line.startswith("  U1 ") and not line.endswith("o=0")]
+    assert len(cond) >= 5
+    us = _draws(b, 61)
+    rex, rnn, roc = oracle_entry(text, paulis, us[0])
+    assert max(abs(r) for r in rex) > 1e-3  # expectations that see the state
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    b.emulate(uniforms=us)
+    ex, nn, oc = b.result(0)
+    assert abs(nn - rnn) < tol
+    assert max(abs(a - r) for a, r in zip(ex, rex)) < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [nc.C128, nc.C64])
+def test_gpu_conditioned_one_position_runs(precision):
+    """One-position runs under an outer condition on the device: circuits of
+    the 10-22 qubit mix that plan them, on tiles smaller than the state,
+    against the oracle."""
+    from ncsim_amd import workloads as W
+    specs = W.batch_circuits(64, 11)
+    tol = 1e-12 if precision == nc.C128 else 2e-5
+    for i, L in ((35, 6), (33, 7), (59, 6), (30, 6)):
+        n, text, _ = specs[i]
+        paulis = [nc.Pauli(n, 1 << q, 0, 0) for q in range(n)] + random_paulis(n, 6, 62)
+        b = _build([text], [paulis], precision, L)
+        us = _draws(b, 63)
+        b.run(uniforms=us)
+        ex, nn, oc = b.result(0)
+        rex, rnn, roc = oracle_entry(text, paulis, us[0])
+        assert abs(nn - rnn) < tol, (i, L)
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < tol, (i, L)
+
+
 @pytest.mark.parametrize("precision,threads,tile_bits", [
     (nc.C128, "512", 7), (nc.C64, "512", 7), (nc.C128, "256", 7), (nc.C64, "256", 7),
     (nc.C128, "256", 11), (nc.C64, "256", 12), (nc.C128, "512", 12)])
@@ -394,6 +444,20 @@ def test_emulated_shapes_match_oracle(low_bits, tiles):
         rex, rnn, roc = oracle_entry(t, ps, us[i])
         assert oc == roc
         assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+
+
+def test_autotune_choice_breaks_near_ties_toward_longer_items():
+    """The fastest candidate wins unless another within 3 % has longer queue
+    items (the measured samples of the trajectory and circuit mixes)."""
+    from ncsim_amd import autotune as AT
+    cfg = lambda b, lo, it: {"tile_bits": b, "low_bits": lo, "tiles_per_item": it}
+    traj = [(cfg(13, 5, 32), 293.56), (cfg(13, 5, 128), 299.62), (cfg(13, 3, 32), 295.47),
+            (cfg(13, 3, 128), 302.07), (cfg(12, 5, 32), 333.75), (cfg(12, 3, 128), 319.68)]
+    assert AT.choose(traj) == cfg(13, 5, 128)
+    mix = [(cfg(13, 5, 32), 38.30), (cfg(13, 5, 128), 66.99), (cfg(12, 5, 32), 33.46),
+           (cfg(12, 3, 32), 31.66), (cfg(12, 3, 128), 96.60)]
+    assert AT.choose(mix) == cfg(12, 3, 32)
+    assert AT.choose([(cfg(12, 3, 32), 10.0), (cfg(12, 3, 128), 10.5)]) == cfg(12, 3, 32)
 
 
 def test_autotune_candidates_and_signature():
