@@ -14,6 +14,7 @@
 #include <cstring>
 #include <future>
 #include <thread>
+#include <unordered_map>
 
 #include "batch.hpp"
 #include "fuse.hpp"
@@ -577,9 +578,12 @@ std::vector<BOpD> group_ops(StepH *s, const std::vector<BOpD> &in, bool drop_slo
 }  // namespace
 
 // Plans one circuit (plus its Paulis) for tiles of at most Lb qubits.
+// one_error_lists: also plan, per step, the op lists keeping one noise slot
+// each (worth their planning and table space when many entries share the
+// program, as noise trajectories do)
 std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<uint64_t> &X,
                                        const std::vector<uint64_t> &Z, const std::vector<uint32_t> &PH,
-                                       ncsv_precision prec, uint32_t Lb, int low_bits) {
+                                       ncsv_precision prec, uint32_t Lb, int low_bits, bool one_error_lists) {
     auto P = std::make_shared<Program>();
     const uint32_t n = c->n;
     P->n = n;
@@ -845,6 +849,7 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     }
     const bool fuse = block_fusion_enabled();
     const bool f32 = prec == NCSV_C64;
+    size_t one_error_budget = (size_t)1 << 20;
     for (StepH &st : P->steps) {
         auto plan = [&](const std::vector<BOpD> &all, bool drop_slots) {
             // phases on outer qubits only multiply whole tiles and commute with
@@ -876,14 +881,23 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         st.gops = plan(st.ops, false);
         st.gopsc = plan(st.ops, true);
         // one list per noise slot keeping only that slot: a step where one
-        // error was drawn runs it, with the other slots' sites fused over
-        if (st.slots.size() >= 2 && st.slots.size() <= 256) {
+        // error was drawn runs it, with the other slots' sites fused over;
+        // at most 2^20 complex coefficients of such lists per program
+        if (one_error_lists && st.slots.size() >= 2 && st.slots.size() <= 256) {
+            const size_t before = st.coef.size();
             st.gops1.resize(st.slots.size());
             for (size_t k = 0; k < st.slots.size(); k++) {
                 std::vector<BOpD> one;
                 for (const BOpD &o : st.ops)
                     if (o.kind != BK_NSLOT || o.a == st.slots[k]) one.push_back(o);
                 st.gops1[k] = plan(one, false);
+            }
+            if (st.coef.size() - before > one_error_budget) {
+                st.gops1.clear();
+                st.coef.resize(before);
+                one_error_budget = 0;
+            } else {
+                one_error_budget -= st.coef.size() - before;
             }
         }
     }
@@ -894,8 +908,8 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             const StepH &st = P->steps[i];
             int cnt[15] = {0};
             for (const BOpD &o : st.gopsc) cnt[o.kind]++;
-            fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu:", i, st.L, st.flags, st.gops.size(),
-                    st.gopsc.size(), st.slots.size());
+            fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu one-error-lists=%zu:", i, st.L,
+                    st.flags, st.gops.size(), st.gopsc.size(), st.slots.size(), st.gops1.size());
             for (int k = 0; k < 15; k++)
                 if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
             fprintf(stderr, "\n");
@@ -1032,6 +1046,18 @@ void plan_all(ncsv_batch *b) {
             return e.prog >= 0;
         }))
         return;
+    // entries sharing each program to be built (single-error lists pay only
+    // for programs that many entries run)
+    std::vector<uint32_t> sharers(todo.size(), 1);
+    {
+        std::unordered_map<size_t, size_t> at;
+        for (size_t k = 0; k < todo.size(); k++) at[todo[k]] = k;
+        for (const auto &e : b->entries)
+            if (e.prog <= -2) {
+                const auto it = at.find((size_t)(-2 - e.prog));
+                if (it != at.end()) sharers[it->second]++;
+            }
+    }
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::shared_ptr<Program>> made(todo.size());
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1044,7 +1070,7 @@ void plan_all(ncsv_batch *b) {
                 const size_t k = next.fetch_add(1);
                 if (k >= todo.size()) return;
                 const auto &e = b->entries[todo[k]];
-                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb, b->low_bits);
+                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb, b->low_bits, sharers[k] >= 16);
             }
         }));
     }
@@ -1294,6 +1320,14 @@ Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *
             ch.recs.push_back(r);
             ch.total += P.tiles(d) >> r.lgt;
         }
+    }
+    if (getenv("NCSV_BATCH_DUMP")) {
+        size_t err = 0, one = 0;
+        for (const BRecD &r : ch.recs) {
+            err += (r.flags & RF_ERR) != 0;
+            one += (r.flags & RF_ONE) != 0;
+        }
+        fprintf(stderr, "records=%zu with-errors=%zu one-error=%zu\n", ch.recs.size(), err, one);
     }
     BRecD sentinel{};
     sentinel.first = ch.total;

@@ -400,6 +400,45 @@ def test_gpu_conditioned_one_position_runs(precision):
         assert max(abs(a - r) for a, r in zip(ex, rex)) < tol, (i, L)
 
 
+def test_one_error_lists_for_shared_noisy_programs(capfd, monkeypatch):
+    """Sixteen trajectories of one noisy 8-qubit circuit (7-qubit tiles) share
+    a program, which then
+    also plans, per step, the op lists keeping one noise slot each; a step
+    where exactly one error was drawn runs that list.  Every trajectory
+    equals the oracle with its own draws (the kernel's source on the host,
+    and the FP64 emulation)."""
+    monkeypatch.setenv("NCSV_BATCH_DUMP", "1")
+    text = random_circuit(8, 4, 71, measure=True, noise=False)
+    lines = text.strip().splitlines()
+    noisy = [lines[0]]
+    for k, line in enumerate(lines[1:]):
+        noisy.append(line)
+        if k % 3 == 0 and not line.startswith(("M", "R", "MPP")):
+            noisy.append(f"DEPOLARIZE1(0.03) {k % 8}")
+    text = "\n".join(noisy) + "\n"
+    paulis = random_paulis(8, 3, 72)
+    b = nc.HipBatch(nc.C64)
+    b.set_tile_bits(7)
+    c = nc.Circuit.parse(text)
+    for _ in range(16):
+        b.add(c, paulis)
+    capfd.readouterr()
+    b.plan()
+    assert any("one-error-lists=" in line and "one-error-lists=0" not in line
+               for line in capfd.readouterr().err.splitlines())
+    us = _draws(b, 73)
+    for run in (b.simulate, b.emulate):
+        run(uniforms=us)
+        stats = [line for line in capfd.readouterr().err.splitlines() if line.startswith("records=")]
+        assert stats and int(stats[-1].split("one-error=")[1]) > 0
+        for i in range(16):
+            ex, nn, oc = b.result(i)
+            rex, rnn, roc = oracle_entry(text, paulis, us[i])
+            assert oc == roc, i
+            assert abs(nn - rnn) < 2e-5
+            assert max(abs(a - r) for a, r in zip(ex, rex)) < 2e-5, i
+
+
 @pytest.mark.parametrize("precision,threads,tile_bits", [
     (nc.C128, "512", 7), (nc.C64, "512", 7), (nc.C128, "256", 7), (nc.C64, "256", 7),
     (nc.C128, "256", 11), (nc.C64, "256", 12), (nc.C128, "512", 12)])
