@@ -578,12 +578,13 @@ std::vector<BOpD> group_ops(StepH *s, const std::vector<BOpD> &in, bool drop_slo
 }  // namespace
 
 // Plans one circuit (plus its Paulis) for tiles of at most Lb qubits.
-// one_error_lists: also plan, per step, the op lists keeping one noise slot
-// each (worth their planning and table space when many entries share the
-// program, as noise trajectories do)
+// one_error_budget: complex coefficients the program may spend on op lists
+// keeping one noise slot each, planned per step while the budget lasts
+// (worth their planning and table space when many entries share the
+// program, as noise trajectories do; 0: none)
 std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<uint64_t> &X,
                                        const std::vector<uint64_t> &Z, const std::vector<uint32_t> &PH,
-                                       ncsv_precision prec, uint32_t Lb, int low_bits, bool one_error_lists) {
+                                       ncsv_precision prec, uint32_t Lb, int low_bits, size_t one_error_budget) {
     auto P = std::make_shared<Program>();
     const uint32_t n = c->n;
     P->n = n;
@@ -849,7 +850,6 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
     }
     const bool fuse = block_fusion_enabled();
     const bool f32 = prec == NCSV_C64;
-    size_t one_error_budget = (size_t)1 << 20;
     for (StepH &st : P->steps) {
         auto plan = [&](const std::vector<BOpD> &all, bool drop_slots) {
             // phases on outer qubits only multiply whole tiles and commute with
@@ -881,9 +881,8 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         st.gops = plan(st.ops, false);
         st.gopsc = plan(st.ops, true);
         // one list per noise slot keeping only that slot: a step where one
-        // error was drawn runs it, with the other slots' sites fused over;
-        // at most 2^20 complex coefficients of such lists per program
-        if (one_error_lists && st.slots.size() >= 2 && st.slots.size() <= 256) {
+        // error was drawn runs it, with the other slots' sites fused over
+        if (one_error_budget && st.slots.size() >= 2 && st.slots.size() <= 256) {
             const size_t before = st.coef.size();
             st.gops1.resize(st.slots.size());
             for (size_t k = 0; k < st.slots.size(); k++) {
@@ -895,7 +894,6 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
             if (st.coef.size() - before > one_error_budget) {
                 st.gops1.clear();
                 st.coef.resize(before);
-                one_error_budget = 0;
             } else {
                 one_error_budget -= st.coef.size() - before;
             }
@@ -1046,8 +1044,10 @@ void plan_all(ncsv_batch *b) {
             return e.prog >= 0;
         }))
         return;
-    // entries sharing each program to be built (single-error lists pay only
-    // for programs that many entries run)
+    // entries sharing each program to be built: single-error lists pay only
+    // for programs that 16 or more entries run; each such program may spend
+    // 2^22 complex coefficients (64 MiB) on them, and all of them together
+    // 2^24
     std::vector<uint32_t> sharers(todo.size(), 1);
     {
         std::unordered_map<size_t, size_t> at;
@@ -1058,6 +1058,8 @@ void plan_all(ncsv_batch *b) {
                 if (it != at.end()) sharers[it->second]++;
             }
     }
+    const size_t nshared = (size_t)std::count_if(sharers.begin(), sharers.end(), [](uint32_t v) { return v >= 16; });
+    const size_t budget = nshared ? std::min<size_t>((size_t)1 << 22, ((size_t)1 << 24) / nshared) : 0;
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::shared_ptr<Program>> made(todo.size());
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1070,7 +1072,7 @@ void plan_all(ncsv_batch *b) {
                 const size_t k = next.fetch_add(1);
                 if (k >= todo.size()) return;
                 const auto &e = b->entries[todo[k]];
-                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb, b->low_bits, sharers[k] >= 16);
+                made[k] = build_program(e.c, e.x, e.z, e.ph, b->prec, Lb, b->low_bits, sharers[k] >= 16 ? budget : 0);
             }
         }));
     }
