@@ -102,7 +102,9 @@ ncsv_status ncsv_clone(const ncsv_state *s, ncsv_state **out);
 /* QuantumState::num_qubits (mod.rs:46). */
 ncsv_status ncsv_num_qubits(const ncsv_state *s, uint32_t *n);
 ncsv_status ncsv_get_precision(const ncsv_state *s, ncsv_precision *prec);
-/* StateVector::basis_state (vector.rs:29-34). */
+/* StateVector::basis_state (vector.rs:29-34).  The amplitudes are written
+ * lazily: by the next ncsv_run's first sweep when it starts with gates,
+ * otherwise by whichever call reads the state first; results are the same. */
 ncsv_status ncsv_set_basis(ncsv_state *s, uint64_t index);
 /* StateVector::plus_state (vector.rs:36-42): every amplitude sqrt2^-n. */
 ncsv_status ncsv_set_plus(ncsv_state *s);

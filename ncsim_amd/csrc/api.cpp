@@ -166,6 +166,15 @@ namespace ncsv {
 ncsv_status guard_status(const std::function<void()> &f) { return guard(f); }
 }  // namespace ncsv
 
+namespace ncsv {
+void materialize(ncsv_state *s) {
+    if (s->pending_basis < 0) return;
+    const uint64_t b = (uint64_t)s->pending_basis;
+    s->pending_basis = -1;
+    dev::fill_basis(s, b);
+}
+}  // namespace ncsv
+
 extern "C" {
 
 size_t ncsv_last_error(char *buf, size_t cap) {
@@ -220,6 +229,7 @@ ncsv_status ncsv_clone(const ncsv_state *src, ncsv_state **out) {
         need(src, "state");
         need(out, "out");
         std::lock_guard<std::recursive_mutex> lk(src->mu);
+        materialize(const_cast<ncsv_state *>(src));
         ncsv_state *d = alloc_state(src->n, src->prec);
         try {
             dev::sync(src);
@@ -254,8 +264,9 @@ ncsv_status ncsv_set_basis(ncsv_state *s, uint64_t index) {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
         if (index >> s->n) fail(NCSV_INVALID, "basis index out of range");
-        dev::fill_basis(s, index);
-        dev::sync(s);
+        // written by the next run's first sweep, or before anything else reads the state
+        s->pending_basis = (int64_t)index;
+        s->lscale = 1.0;
     });
 }
 
@@ -263,6 +274,7 @@ ncsv_status ncsv_set_plus(ncsv_state *s) {
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        s->pending_basis = -1;  // overwritten
         dev::fill_const(s, sqrt2_pow(-(int64_t)s->n).real());
         dev::sync(s);
     });
@@ -275,6 +287,7 @@ ncsv_status ncsv_set_amplitudes(ncsv_state *s, const double *amp, uint64_t len) 
         std::lock_guard<std::recursive_mutex> lk(s->mu);
         if (len != (1ull << s->n))
             fail(NCSV_INVALID, std::to_string(len) + " amplitudes for " + std::to_string(s->n) + " qubits");
+        s->pending_basis = -1;  // overwritten
         dev::upload(s, amp);
     });
 }
@@ -284,6 +297,7 @@ ncsv_status ncsv_to_vector(const ncsv_state *s, double *out, uint64_t len) {
         need(s, "state");
         need(out, "out");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         if (len != (1ull << s->n)) fail(NCSV_INVALID, "to_vector: length must be 2^n");
         dev::download(s, out);
     });
@@ -294,6 +308,7 @@ ncsv_status ncsv_amplitude(const ncsv_state *s, uint64_t index, double out[2]) {
         need(s, "state");
         need(out, "out");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         if (s->n < 64 && (index >> s->n)) fail(NCSV_INVALID, "basis index out of range");
         dev::read_amp(s, index, out);
     });
@@ -305,6 +320,7 @@ ncsv_status ncsv_apply_gate(ncsv_state *s, ncsv_gate gate, const int64_t *ip, co
         need(s, "state");
         if (nq) need(qubits, "qubits");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         const GateDesc g = make_gate(gate, ip, fp, nq);
         const LOp op = lower_gate(g, qubits, nq, s->n);
         dev::apply_lop(s, op);
@@ -317,6 +333,7 @@ ncsv_status ncsv_apply_matrix(ncsv_state *s, const double *m, const uint32_t *qu
         need(m, "matrix");
         if (k) need(qubits, "qubits");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         check_qubits(qubits, k, s->n);
         if (k > 12) fail(NCSV_UNSUPPORTED, "dense matrix on more than 12 qubits");
         LOp op;
@@ -337,6 +354,7 @@ ncsv_status ncsv_apply_monomial(ncsv_state *s, const uint32_t *qubits, uint32_t 
         need(phases, "phases");
         if (k) need(qubits, "qubits");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         check_qubits(qubits, k, s->n);
         if (k > 4) fail(NCSV_UNSUPPORTED, "monomial gate on more than 4 qubits");
         const uint32_t d = 1u << k;
@@ -361,6 +379,7 @@ ncsv_status ncsv_apply_pauli(ncsv_state *s, uint64_t x, uint64_t z, uint32_t pha
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         check_register(s, x, z, "apply_pauli");
         dev::apply_pauli(s, x, z, phase & 3);
     });
@@ -371,6 +390,7 @@ ncsv_status ncsv_norm_sq(const ncsv_state *s, double out[2]) {
         need(s, "state");
         need(out, "out");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         ncsv_state *m = const_cast<ncsv_state *>(s);
         out[0] = dev::norm_sq(m) * lsq(s);
         out[1] = 0.0;
@@ -392,6 +412,7 @@ ncsv_status ncsv_expectation_batch(const ncsv_state *s, uint32_t count, const ui
             need(out, "out");
         }
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         for (uint32_t i = 0; i < count; i++) check_register(s, x[i], z[i], "expectation");
         ncsv_state *m = const_cast<ncsv_state *>(s);
         dev::expectations(m, count, x, z, phase, out);
@@ -444,6 +465,8 @@ ncsv_status ncsv_inner(const ncsv_state *a, const ncsv_state *b, double out[2]) 
             l2.lock();
             l1.lock();
         }
+        materialize(const_cast<ncsv_state *>(a));
+        materialize(const_cast<ncsv_state *>(b));
         dev::inner(const_cast<ncsv_state *>(a), const_cast<ncsv_state *>(b), out);
         out[0] *= a->lscale * b->lscale;
         out[1] *= a->lscale * b->lscale;
@@ -465,6 +488,8 @@ ncsv_status ncsv_compare(const ncsv_state *a, const ncsv_state *b, double out[5]
             l2.lock();
             l1.lock();
         }
+        materialize(const_cast<ncsv_state *>(a));
+        materialize(const_cast<ncsv_state *>(b));
         dev::compare(const_cast<ncsv_state *>(a), const_cast<ncsv_state *>(b), out);
     });
 }
@@ -474,6 +499,7 @@ ncsv_status ncsv_probability(const ncsv_state *s, uint64_t index, double *out) {
         need(s, "state");
         need(out, "out");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         if (s->n < 64 && (index >> s->n)) fail(NCSV_INVALID, "basis index out of range");
         double a[2];
         dev::read_amp(s, index, a);
@@ -486,6 +512,7 @@ ncsv_status ncsv_project(ncsv_state *s, uint64_t x, uint64_t z, uint32_t phase, 
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         const double w = do_project(s, x, z, phase & 3, outcome ? 1 : 0);
         if (weight) {
             weight[0] = w;
@@ -499,6 +526,7 @@ ncsv_status ncsv_measure(ncsv_state *s, uint64_t x, uint64_t z, uint32_t phase, 
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         const int o = do_measure(s, x, z, phase & 3, rng, ctx);
         if (outcome) *outcome = o;
     });
@@ -508,6 +536,7 @@ ncsv_status ncsv_reset(ncsv_state *s, uint32_t qubit, ncsv_uniform_fn rng, void 
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         check_qubits(&qubit, 1, s->n);
         if (do_measure(s, 0, 1ull << qubit, 0, rng, ctx)) dev::apply_pauli(s, 1ull << qubit, 0, 0);
     });
@@ -518,6 +547,7 @@ ncsv_status ncsv_scale(ncsv_state *s, const double c[2]) {
         need(s, "state");
         need(c, "c");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         do_scale(s, cd(c[0], c[1]));
     });
 }
@@ -526,6 +556,7 @@ ncsv_status ncsv_normalize(ncsv_state *s) {
     return guard([&] {
         need(s, "state");
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         const double nn = dev::norm_sq(s) * lsq(s);
         if (nn <= 0.0) fail(NCSV_ZERO_STATE, "state vanished: normalize");
         const double l = -std::log2(nn);
@@ -546,6 +577,7 @@ ncsv_status ncsv_sample(const ncsv_state *s, uint64_t shots, const double *unifo
             need(out, "out");
         }
         std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
         dev::sample(const_cast<ncsv_state *>(s), shots, uniforms, out);
     });
 }
@@ -571,6 +603,8 @@ ncsv_status ncsv_tensor(const ncsv_state *a, const ncsv_state *b, ncsv_state **o
         std::lock_guard<std::recursive_mutex> l1(a->mu);
         std::unique_lock<std::recursive_mutex> l2(b->mu, std::defer_lock);
         if (a != b) l2.lock();
+        materialize(const_cast<ncsv_state *>(a));
+        materialize(const_cast<ncsv_state *>(b));
         ncsv_state *o = alloc_state(n, a->prec);
         try {
             dev::kron(const_cast<ncsv_state *>(a), const_cast<ncsv_state *>(b), o);
@@ -740,6 +774,7 @@ ncsv_status ncsv_run(ncsv_state *s, const ncsv_circuit *cc, ncsv_uniform_fn rng,
                 i = j;
                 continue;
             }
+            materialize(s);  // measurements, resets and noise read the amplitudes
             switch (op.kind) {
             case CKind::Measure: {
                 const int o = do_measure(s, op.x, op.z, op.ph, rng, ctx);

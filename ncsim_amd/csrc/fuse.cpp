@@ -502,10 +502,12 @@ struct Plan {
     std::vector<SegItem> items;
     std::vector<size_t> prog_off;  // offset of each tile item's program in dbuf
     std::vector<void *> kern;      // specialized kernel per tile item (JIT plans)
+    void *kern_from_basis = nullptr;  // the first item's kernel started from a basis state (made on first use)
     void *dbuf = nullptr;
     ~Plan() {
         for (void *k : kern)
             if (k) jit::release(k);
+        if (kern_from_basis) jit::release(kern_from_basis);
         if (dbuf) (void)hipFree(dbuf);
     }
 };
@@ -589,6 +591,7 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
 void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
     const TileGeom geo = dev::tile_geom(s->prec);
     if (s->fusion == 0 || s->n < geo.L()) {
+        materialize(s);
         for (size_t t = i; t < j; t++) {
             const COp &op = c->ops[t];
             if (op.kind != CKind::Gate) continue;
@@ -653,6 +656,27 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
     }
     for (size_t k = 0; k < plan->items.size(); k++) {
         const SegItem &it = plan->items[k];
+        if (s->pending_basis >= 0) {
+            if (k == 0 && it.tile && plan->kern[0]) {
+                // from a deferred basis state: the first sweep computes its
+                // input in registers instead of reading a filled state
+                void *kb;
+                {
+                    std::lock_guard<std::mutex> lk(c->mu);
+                    if (!plan->kern_from_basis) {
+                        std::vector<double> cb;
+                        plan->kern_from_basis = jit::prepare_named(
+                            jit::generate(it.sw, s->n, geo, s->prec == NCSV_C128, &cb, true), "ncsv_sweep_from_basis");
+                    }
+                    kb = plan->kern_from_basis;
+                }
+                const uint64_t b = (uint64_t)s->pending_basis;
+                s->pending_basis = -1;
+                jit::launch_from_basis(s, kb, geo.L(), geo.LT, (const char *)plan->dbuf + plan->prog_off[0], b);
+                continue;
+            }
+            materialize(s);
+        }
         if (it.tile && plan->kern[k])
             jit::launch(s, plan->kern[k], geo.L(), geo.LT, (const char *)plan->dbuf + plan->prog_off[k]);
         else if (it.tile)

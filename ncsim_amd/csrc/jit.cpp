@@ -445,7 +445,8 @@ bool diag_run_pays(const std::vector<const TileOpH *> &run, uint32_t NR) {
 
 }  // namespace
 
-std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef) {
+std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef,
+                     bool from_basis) {
     Gen g;
     g.coef = coef;
     g.literal = use_literals();
@@ -477,7 +478,9 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
       << ")) & " << ((1u << S) - 1) << "u); }\n";
     o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
-      << ") void ncsv_sweep(V *__restrict__ amp, const T *__restrict__ c) {\n";
+      << ") void " << (from_basis ? "ncsv_sweep_from_basis" : "ncsv_sweep")
+      << "(V *__restrict__ amp, const T *__restrict__ c" << (from_basis ? ", unsigned long long binit" : "")
+      << ") {\n";
     if (lds) o << "  __shared__ V lds[" << (1u << g.L) << "];\n";
     o << "  const unsigned tid = threadIdx.x;\n  const unsigned long long b = blockIdx.x;\n";
     o << "  const unsigned long long base = " << deposit("b", g.oq, true) << ";\n";
@@ -495,7 +498,13 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
             if ((r >> j) & 1) off |= 1ull << sw.tq[g.LT + j];
         return off;
     };
-    for (uint32_t r = 0; r < g.NR; r++) o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
+    for (uint32_t r = 0; r < g.NR; r++) {
+        if (from_basis)
+            o << "  V a" << r << "; a" << r << ".x = (T)((gm | " << roff(r) << "ull) == binit ? 1 : 0); a" << r
+              << ".y = (T)0;\n";
+        else
+            o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
+    }
     uint32_t cur = memr;
     for (auto &St : sw.stages) {
         if (St.rmask_pos != cur) {
@@ -796,6 +805,21 @@ void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dc
     s->sweeps++;
 }
 
+void launch_from_basis(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef, uint64_t binit) {
+    auto &f = *(std::shared_future<std::shared_ptr<Entry>> *)handle;
+    const std::shared_ptr<Entry> e = f.get();
+    if (!e->fn) fail(NCSV_DEVICE, e->err);
+    void *amp = s->amp;
+    const void *c = dcoef;
+    unsigned long long b = binit;
+    void *args[] = {&amp, &c, &b};
+    const uint64_t tiles = 1ull << (s->n - L);
+    KTimer kt(s, "sweep_jit_from_basis", (double)s->amp_bytes());
+    check_hip(hipModuleLaunchKernel(e->fn, (uint32_t)tiles, 1, 1, 1u << LT, 1, 1, 0, s->stream, args, nullptr),
+              "hipModuleLaunchKernel(sweep from a basis state)");
+    s->sweeps++;
+}
+
 void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_t grid, double *dpart) {
     auto &f = *(std::shared_future<std::shared_ptr<Entry>> *)handle;
     const std::shared_ptr<Entry> e = f.get();
@@ -807,8 +831,6 @@ void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_
     check_hip(hipModuleLaunchKernel(e->fn, grid, 1, 1, 1u << LT, 1, 1, 0, s->stream, args, nullptr),
               "hipModuleLaunchKernel(expect)");
 }
-
-void *prepare_named(const std::string &src, const char *fname);
 
 bool available() {
     static int ok = -1;

@@ -9,14 +9,23 @@
 namespace ncsv {
 namespace jit {
 // HIP source of one tile sweep; appends the sweep's coefficients to *coef.
-std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef);
+// from_basis: the sweep starts from the basis state |binit> (a kernel
+// argument) computed in registers instead of loaded, with the same
+// coefficients in the same order.
+std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef,
+                     bool from_basis = false);
 // hiprtc compile for `arch`; empty on failure (message in *log).
 std::vector<char> compile_to_code(const std::string &src, const std::string &arch, std::string *log);
 // Starts (or finds) the compile of `src`; returns an opaque handle.
 void *prepare(const std::string &src);
+// The same for a kernel named fname (sweeps from a basis state: ncsv_sweep_from_basis).
+void *prepare_named(const std::string &src, const char *fname);
 void release(void *handle);
 // Launches a prepared sweep on the state's stream (waits for its compile).
 void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef);
+// Launches a prepared from_basis sweep: the state becomes its result applied
+// to |binit> (no amplitude is read).
+void launch_from_basis(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef, uint64_t binit);
 // false when NCSV_JIT=0.
 bool available();
 

@@ -128,6 +128,10 @@ struct ncsv_state {
     int device = 0;
     void *amp = nullptr;  // 2^n complex amplitudes (double2 or float2)
     double lscale = 1.0;  // logical amplitude = lscale * stored (complex64 form)
+    // set_basis deferred: the state is |pending_basis> but the amplitudes are
+    // not written yet (-1: they are); a run's first specialized sweep computes
+    // them in registers, anything else writes them first (materialize)
+    int64_t pending_basis = -1;
     hipStream_t stream = nullptr;
     // reduction partials and small tables
     double *d_part = nullptr;
@@ -146,6 +150,11 @@ struct ncsv_state {
     size_t amp_bytes() const { return (size_t(1) << n) * (prec == NCSV_C128 ? 16 : 8); }
     size_t elem_bytes() const { return prec == NCSV_C128 ? 16 : 8; }
 };
+
+namespace ncsv {
+// Writes a deferred basis state into the amplitudes (no-op when none).
+void materialize(ncsv_state *s);
+}  // namespace ncsv
 
 namespace ncsv {
 namespace dev {

@@ -233,6 +233,39 @@ def run_both(text, seed, fusion=None):
     return o, d, ro, rd
 
 
+@pytest.mark.parametrize("precision", [H.C128, H.C64])
+def test_deferred_basis_state(precision):
+    """set_basis defers its fill: a run's first specialized sweep (22 qubits)
+    starts from the basis state in registers, and any other reader writes the
+    state first.  Every path against the oracle started from the same basis
+    state."""
+    n = 22
+    text = W.rotation_brickwork(n, 6, 9)
+    tol = 1e-12 if precision == H.C128 else 2e-5
+    d = H.HipStateVector.zero_state(n, precision)
+    for b in (0, 0x2A5F3, (1 << n) - 1):
+        o = StateVector.basis_state(n, b)
+        o.run(OCircuit.parse(text), SplitMix64(0))
+        want = o.to_vector()
+        # run straight after set_basis: the sweep from the basis state
+        d.set_basis(b)
+        d.run(H.Circuit.parse(text))
+        assert np.abs(d.to_vector() - want).max() < tol, b
+        # readers before a run write the basis state first
+        d.set_basis(b)
+        assert abs(d.amplitude(b) - 1.0) < 1e-15 and abs(d.norm_sq() - 1.0) < 1e-15
+        d.run(H.Circuit.parse(text))
+        assert np.abs(d.to_vector() - want).max() < tol, b
+        d.set_basis(b)
+        x = np.zeros(1 << n)
+        x[b] = 1.0
+        assert np.array_equal(np.abs(d.to_vector()), x)
+        # a run whose first op is a measurement writes it first as well
+        d.set_basis(b)
+        rec = d.run(H.Circuit.parse(f"QUBITS {n}\nM 3\n" + text.split("\n", 1)[1]), SplitMix64(1))
+        assert rec == [bool((b >> 3) & 1)]
+
+
 def test_measurements_resets_and_noise_follow_the_same_draws():
     """mod.rs:113-165: the records and final states agree when both sides
     draw from the same generator."""
