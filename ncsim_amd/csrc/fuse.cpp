@@ -638,6 +638,14 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
                 }
                 total = host.size();
             }
+            // the first sweep's form for runs from a deferred basis state,
+            // compiled alongside the others
+            if (!plan->items.empty() && plan->items[0].tile) {
+                std::vector<double> cb;
+                plan->kern_from_basis = jit::prepare_named(
+                    jit::generate(plan->items[0].sw, s->n, geo, s->prec == NCSV_C128, &cb, true),
+                    "ncsv_sweep_from_basis");
+            }
         } else {
             for (auto &it : plan->items) {
                 plan->prog_off.push_back(total);
@@ -659,7 +667,8 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         if (s->pending_basis >= 0) {
             if (k == 0 && it.tile && plan->kern[0]) {
                 // from a deferred basis state: the first sweep computes its
-                // input in registers instead of reading a filled state
+                // input in registers instead of reading a filled state (its
+                // kernel is made with the plan; made here for a plan without it)
                 void *kb;
                 {
                     std::lock_guard<std::mutex> lk(c->mu);
