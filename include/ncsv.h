@@ -94,7 +94,8 @@ ncsv_status ncsv_device_arch(char *buf, size_t cap);
 
 /* ---- lifetime and construction (vector.rs:24-64) ------------------------ */
 
-/* StateVector::zero_state (vector.rs:25-27).  n <= 34 (device memory). */
+/* StateVector::zero_state (vector.rs:25-27).  n <= 34 (device memory).  The
+ * amplitudes are written lazily, as after ncsv_set_basis. */
 ncsv_status ncsv_create(uint32_t n, ncsv_precision prec, ncsv_state **out);
 ncsv_status ncsv_destroy(ncsv_state *s);
 /* Clone (derive(Clone), vector.rs:15): a device-to-device copy. */

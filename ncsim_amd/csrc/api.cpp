@@ -209,13 +209,9 @@ ncsv_status ncsv_create(uint32_t n, ncsv_precision prec, ncsv_state **out) {
     return guard([&] {
         need(out, "out");
         ncsv_state *s = alloc_state(n, prec);
-        try {
-            dev::fill_basis(s, 0);
-            dev::sync(s);
-        } catch (...) {
-            free_state(s);
-            throw;
-        }
+        // |0...0>, written lazily as after set_basis
+        s->pending_basis = 0;
+        s->lscale = 1.0;
         *out = s;
     });
 }
