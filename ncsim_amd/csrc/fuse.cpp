@@ -179,8 +179,10 @@ std::vector<uint32_t> need_order(const std::vector<Cls> &cl, const std::vector<u
 
 // Greedy choice of `cap` qubits (starting from `fixed`, drawn from `pool`)
 // that admit the most ops of `rem`.
+// variant > 0: the candidate qubits in a permuted order (a fixed function of
+// variant and the set so far), so ties and near-equal choices go elsewhere
 uint64_t choose_set(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem, uint64_t fixed, uint64_t pool,
-                    uint32_t cap) {
+                    uint32_t cap, uint32_t variant = 0) {
     uint64_t T = fixed;
     size_t cur = 0;
     pick(cl, rem, T, nullptr, &cur);
@@ -197,6 +199,18 @@ uint64_t choose_set(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem
         for (uint64_t a = cand; a; a &= a - 1) {
             const uint32_t q = (uint32_t)__builtin_ctzll(a);
             if (std::find(cands.begin(), cands.end(), q) == cands.end()) cands.push_back(q);
+        }
+        if (variant) {
+            // Fisher-Yates with a SplitMix64 stream seeded by (variant, T)
+            uint64_t x = (uint64_t)variant * 0x9E3779B97F4A7C15ull + T;
+            for (size_t i = cands.size(); i > 1; i--) {
+                x += 0x9E3779B97F4A7C15ull;
+                uint64_t z = x;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                std::swap(cands[i - 1], cands[z % i]);
+            }
         }
         for (uint32_t q : cands) {
             size_t s = 0;
@@ -472,13 +486,13 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
 }
 
 std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t> rem, uint32_t n,
-                                const TileGeom &geo, uint64_t first_fixed = 0) {
+                                const TileGeom &geo, uint64_t first_fixed = 0, uint32_t variant = 0) {
     std::vector<SweepH> out;
     const uint32_t L = geo.L();
     const uint64_t fixed = (1ull << std::min(geo.c, n)) - 1;
     const uint64_t pool = n >= 64 ? ~0ull : (1ull << n) - 1;
     while (!rem.empty()) {
-        const uint64_t T = choose_set(cl, rem, fixed | (out.empty() ? first_fixed : 0), pool, L);
+        const uint64_t T = choose_set(cl, rem, fixed | (out.empty() ? first_fixed : 0), pool, L, variant);
         std::vector<uint32_t> got;
         pick(cl, rem, T, &got, nullptr);
         if (got.empty()) fail(NCSV_INVALID, "sweep planner made no progress");
@@ -562,7 +576,21 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
     std::vector<uint32_t> run;
     auto flush = [&]() {
         if (run.empty()) return;
-        for (auto &sw : plan_sweeps(cl, run, n, geo)) {
+        // the greedy tile choice, and on states of 2^20 amplitudes or more
+        // (where a sweep costs milliseconds) 23 permuted variants of it: the
+        // plan with the fewest sweeps, then the fewest register stages
+        std::vector<SweepH> best = plan_sweeps(cl, run, n, geo);
+        auto stages = [](const std::vector<SweepH> &p) {
+            size_t k = 0;
+            for (const SweepH &sw : p) k += sw.stages.size();
+            return k;
+        };
+        if (n >= 20)
+            for (uint32_t v = 1; v < 24; v++) {
+                std::vector<SweepH> c = plan_sweeps(cl, run, n, geo, 0, v);
+                if (c.size() < best.size() || (c.size() == best.size() && stages(c) < stages(best))) best.swap(c);
+            }
+        for (auto &sw : best) {
             SegItem it;
             it.tile = true;
             it.sw = std::move(sw);
