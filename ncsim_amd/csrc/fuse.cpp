@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <tuple>
 
 namespace ncsv {
 
@@ -578,18 +579,29 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
         if (run.empty()) return;
         // the greedy tile choice, and on states of 2^20 amplitudes or more
         // (where a sweep costs milliseconds) 23 permuted variants of it: the
-        // plan with the fewest sweeps, then the fewest register stages
+        // plan with the fewest sweeps, then the least estimated time (a sweep
+        // costs the larger of its ops and geo.floor_ops, the ops its memory
+        // traffic takes as long as), then the fewest register stages
         std::vector<SweepH> best = plan_sweeps(cl, run, n, geo);
-        auto stages = [](const std::vector<SweepH> &p) {
-            size_t k = 0;
-            for (const SweepH &sw : p) k += sw.stages.size();
-            return k;
+        auto score = [&](const std::vector<SweepH> &p) {
+            size_t t = 0, k = 0;
+            for (const SweepH &sw : p) {
+                t += std::max<size_t>(sw.ops.size(), geo.floor_ops);
+                k += sw.stages.size();
+            }
+            return std::make_tuple(p.size(), t, k);
         };
-        if (n >= 20)
+        if (n >= 20) {
+            auto bs = score(best);
             for (uint32_t v = 1; v < 24; v++) {
                 std::vector<SweepH> c = plan_sweeps(cl, run, n, geo, 0, v);
-                if (c.size() < best.size() || (c.size() == best.size() && stages(c) < stages(best))) best.swap(c);
+                const auto cs = score(c);
+                if (cs < bs) {
+                    best.swap(c);
+                    bs = cs;
+                }
             }
+        }
         for (auto &sw : best) {
             SegItem it;
             it.tile = true;

@@ -62,6 +62,8 @@ struct TileGeom {
     bool pipe = false;              // load the next op while applying the current one
     bool merge_diag = false;        // multiply one-qubit diagonals into neighbouring 2x2s (saves passes
                                     // where ops cost a pass each; costs VALU where 2x2s have cheap forms)
+    uint32_t floor_ops = 117;       // tile ops whose arithmetic takes as long as a sweep's memory
+                                    // traffic (complex128 on this part; about half for complex64)
     uint32_t L() const { return LT + R; }
     int key() const { return (int)((LT << 12) | (R << 8) | (c << 4) | (pipe ? 1 : 0)); }
 };

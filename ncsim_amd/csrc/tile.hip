@@ -534,6 +534,7 @@ TileGeom tile_geom(ncsv_precision prec) {
         g.LT = 8;
         g.R = 5;
         g.c = 5;
+        g.floor_ops = 58;  // half the bytes, FP32 at the FP64 issue rate
     }
     if (const char *e = getenv("NCSV_TILE_LT")) g.LT = (uint32_t)atoi(e);
     if (const char *e = getenv("NCSV_TILE_R")) g.R = (uint32_t)atoi(e);
