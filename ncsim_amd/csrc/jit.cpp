@@ -845,18 +845,21 @@ bool available() {
 // (nothing done) when some Pauli's X part does not fit one tile's register
 // stage.
 // Most Paulis per group kernel: every Pauli holds two FP64 accumulators per
-// thread.  On 64 random Paulis at 30 qubits, groups of up to 12 gave the
-// fastest step in both precisions (complex128: six passes of 3.5 ms against
-// 3.7 ms at 16; complex64: six of 2.2 ms against five of 3.0 ms, where 16 Paulis
-// take the kernel to 256 registers and one wavefront per SIMD).
-uint32_t group_max() {
+// thread.  On 64 random Paulis at 30 qubits: complex128 tiles (16 registers
+// per thread) group best by 12 (six passes of 3.5 ms; 13 still needs six
+// passes, 16 takes 3.7 ms per pass); complex64 tiles (32 registers per thread,
+// so more X parts share a stage) by 13 (five passes of 2.4 ms against six of
+// 2.2 ms at 12 and five of 3.0 ms at 16, where 16 Paulis take the kernel to
+// 256 registers and one wavefront per SIMD).
+uint32_t group_max(const TileGeom &geo) {
     const char *e = getenv("NCSV_EXPECT_GMAX");
-    return e ? (uint32_t)std::min(16, std::max(1, atoi(e))) : 12u;
+    if (e) return (uint32_t)std::min(16, std::max(1, atoi(e)));
+    return geo.R >= 5 ? 13u : 12u;
 }
 
 bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
                  const uint32_t *ph, std::vector<ExpGroup> *out_groups) {
-    const uint32_t gmax = group_max();
+    const uint32_t gmax = group_max(geo);
     const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, L), LT = geo.LT;
     if (n < L) return false;
     ncsv_state fake;
