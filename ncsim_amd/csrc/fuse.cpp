@@ -517,7 +517,7 @@ struct Plan {
     std::vector<SegItem> items;
     std::vector<size_t> prog_off;  // offset of each tile item's program in dbuf
     std::vector<void *> kern;      // specialized kernel per tile item (JIT plans)
-    void *kern_from_basis = nullptr;  // the first item's kernel started from a basis state (made on first use)
+    void *kern_from_basis = nullptr;  // the first item's kernel started from a basis state (made with the plan)
     void *dbuf = nullptr;
     ~Plan() {
         for (void *k : kern)
