@@ -82,6 +82,7 @@ void free_state(ncsv_state *s) {
     if (s->amp) (void)hipFree(s->amp);
     if (s->d_part) (void)hipFree(s->d_part);
     if (s->d_tab) (void)hipFree(s->d_tab);
+    if (s->d_samp) (void)hipFree(s->d_samp);
     if (s->h_part) (void)hipHostFree(s->h_part);
     fuse_release(s);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -113,7 +114,10 @@ int do_measure(ncsv_state *s, uint64_t x, uint64_t z, uint32_t ph, ncsv_uniform_
     check_register(s, x, z, "measure");
     double e = 0, nn = 0;
     dev::expect_and_norm(s, x, z, ph, &e, &nn);
-    const double p_plus = std::min(1.0, std::max(0.0, (1.0 + e / nn) / 2.0));
+    // f64::clamp keeps a NaN (a zero state: 0/0), so neither certainty test
+    // holds, one uniform is drawn and `u >= NaN` gives outcome 0 (mod.rs:122-128)
+    double p_plus = (1.0 + e / nn) / 2.0;
+    if (!std::isnan(p_plus)) p_plus = std::min(1.0, std::max(0.0, p_plus));
     int outcome;
     if (p_plus > 1.0 - 1e-12)
         outcome = 0;
@@ -169,9 +173,9 @@ ncsv_status guard_status(const std::function<void()> &f) { return guard(f); }
 namespace ncsv {
 void materialize(ncsv_state *s) {
     if (s->pending_basis < 0) return;
-    const uint64_t b = (uint64_t)s->pending_basis;
+    // cleared after the fill went out: a failed fill leaves the basis pending
+    dev::fill_basis(s, (uint64_t)s->pending_basis);
     s->pending_basis = -1;
-    dev::fill_basis(s, b);
 }
 }  // namespace ncsv
 

@@ -1678,8 +1678,9 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
                                 ee += (((c >> ev.d) & 1) ^ ev.sign) ? -W[c] : W[c];
                             }
                             const uint32_t ord = en.m0 + ev.ord;
+                            // NaN (zero state) kept as by f64::clamp: a draw, outcome 0
                             double p_plus = (1.0 + ee / nn) / 2.0;
-                            if (!(p_plus >= 0.0)) p_plus = 0.0;
+                            if (p_plus < 0.0) p_plus = 0.0;
                             if (p_plus > 1.0) p_plus = 1.0;
                             uint32_t o;
                             if (ch.forced[ord] != 0xff)

@@ -141,8 +141,9 @@ NCSV_DEV inline void block_sum(double (&v)[CMAX], uint32_t cnt, double *red, dou
 // QuantumState::measure's decision (mod.rs:113-131) from FP64 sums.
 NCSV_DEV inline uint32_t decide(const double *__restrict__ uni, const uint8_t *__restrict__ forced, uint32_t ordinal,
                                 double nn, double e) {
+    // a NaN (zero state) stays NaN as under f64::clamp: a draw, outcome 0
     double p_plus = (1.0 + e / nn) / 2.0;
-    if (!(p_plus >= 0.0)) p_plus = 0.0;
+    if (p_plus < 0.0) p_plus = 0.0;
     if (p_plus > 1.0) p_plus = 1.0;
     const uint8_t f = forced ? forced[ordinal] : (uint8_t)0xff;
     if (f != 0xff) return f & 1u;

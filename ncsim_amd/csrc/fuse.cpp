@@ -719,9 +719,10 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
                     }
                     kb = plan->kern_from_basis;
                 }
+                // the state counts as written only once the launch went out
                 const uint64_t b = (uint64_t)s->pending_basis;
-                s->pending_basis = -1;
                 jit::launch_from_basis(s, kb, geo.L(), geo.LT, (const char *)plan->dbuf + plan->prog_off[0], b);
+                s->pending_basis = -1;
                 continue;
             }
             materialize(s);

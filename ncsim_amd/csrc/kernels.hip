@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "jit.hpp"
@@ -472,83 +474,109 @@ __global__ __launch_bounds__(TPB) void k_convert(const TI *in, TO *out, uint64_t
 }
 
 // -------------------------------------------------------------- sampling
-// Block weights: one wave per 4096-amplitude block.
+// QuantumState::sample as vector.rs:494-540 computes it, every floating-point
+// operation in the reference's order, so the indices picked are identical
+// given the same draws:
+//  - a block's weight is the left-to-right sum, in index order, of
+//    re*re + im*im (norm_sqr: two rounded products and a rounded sum, no
+//    fused multiply-add), one lane per block;
+//  - the running prefix acc_b and the total are one sequential fold;
+//  - a shot's draw scaled by the total, u, lands in the first block with
+//    u < acc_b + w_b (the last block if none) and walks it from acc_b while
+//    run + |a_idx|^2 <= u, then steps back over amplitudes that are exactly 0.
+
+template <typename V> __device__ __forceinline__ double abs_sq_f64(V v) {
+#pragma clang fp contract(off)
+    const double x = (double)v.x, y = (double)v.y;
+    return x * x + y * y;
+}
+
+// Block weights, one lane per block of `bl` amplitudes, 64 blocks per
+// single-wave workgroup.  The wave loads 128-byte runs (C amplitudes of one
+// block per C lanes), squares them, and passes them through LDS to the lane
+// that owns the block, which adds them in index order.
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_block_weights(const typename VecOf<T>::type *a, uint64_t N, uint64_t bl,
-                                                       uint64_t nblk, double *w) {
+__global__ __launch_bounds__(64) void k_block_weights(const typename VecOf<T>::type *a, uint64_t bl, uint64_t nblk,
+                                                      uint32_t C, double *w) {
+#pragma clang fp contract(off)
     using V = typename VecOf<T>::type;
-    const uint64_t wave = gtid() >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= nblk) return;
+    __shared__ double sq[64 * 16];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t b0 = (uint64_t)blockIdx.x * 64;
+    const uint32_t per = 64 / C;  // blocks per load instruction
     double s = 0.0;
-    for (uint64_t i = lane; i < bl; i += 64) {
-        const V v = a[wave * bl + i];
-        s += (double)v.x * v.x + (double)v.y * v.y;
-    }
-    s = wave_sum(s);
-    if (lane == 0) w[wave] = s;
-}
-
-// Exclusive prefix over the block weights, one block of 1024 threads:
-// chunk sums, a serial scan of the chunk sums, then per-chunk prefixes.
-__global__ void k_scan(const double *w, uint64_t nblk, double *excl, double *total) {
-    __shared__ double cs[1024];
-    const uint64_t T = blockDim.x;
-    const uint64_t chunk = (nblk + T - 1) / T;
-    const uint64_t lo = threadIdx.x * chunk, hi = min(lo + chunk, nblk);
-    double s = 0.0;
-    for (uint64_t i = lo; i < hi; i++) s += w[i];
-    cs[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double acc = 0.0;
-        for (uint64_t i = 0; i < T; i++) {
-            const double v = cs[i];
-            cs[i] = acc;
-            acc += v;
+    for (uint64_t c0 = 0; c0 < bl; c0 += C) {
+        for (uint32_t i = 0; i < C; i++) {
+            const uint32_t bi = i * per + lane / C;
+            const uint64_t b = b0 + bi;
+            double p = 0.0;
+            if (b < nblk) p = abs_sq_f64(a[b * bl + c0 + lane % C]);
+            sq[bi * C + lane % C] = p;
         }
-        *total = acc;
+        __syncthreads();
+        for (uint32_t j = 0; j < C; j++) s += sq[lane * C + j];
+        __syncthreads();
     }
-    __syncthreads();
-    double acc = cs[threadIdx.x];
-    for (uint64_t i = lo; i < hi; i++) {
-        excl[i] = acc;
-        acc += w[i];
-    }
+    if (b0 + lane < nblk) w[b0 + lane] = s;
 }
 
-// One thread per shot: binary search of the block, then the in-block walk of
-// vector.rs:516-535 (never an index of zero weight).
+// The sequential fold of vector.rs:509-537 (acc = hi after each block) and
+// of the total (vector.rs:501): w[b] becomes acc_b + w_b, *total the last.
+__global__ void k_fold(double *w, uint64_t nblk, double *total) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    uint64_t b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+        double v[8];
+        for (int j = 0; j < 8; j++) v[j] = w[b + j];
+        for (int j = 0; j < 8; j++) {
+            acc = acc + v[j];
+            w[b + j] = acc;
+        }
+    }
+    for (; b < nblk; b++) {
+        acc = acc + w[b];
+        w[b] = acc;
+    }
+    *total = acc;
+}
+
+// One thread per shot: u = draw * total; the first block whose inclusive
+// prefix exceeds u (binary search over the non-decreasing prefixes, the
+// same block as the reference's forward scan over the sorted draws), then
+// the in-block walk.  The reference walks a block's shots in draw order
+// from where the previous one stopped; the run values along the walk are
+// the same sequence, so starting each shot at the block start picks the
+// same index.
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_sample(const typename VecOf<T>::type *a, uint64_t N, uint64_t bl,
-                                                uint64_t nblk, const double *w, const double *excl,
-                                                const double *total, const double *us, uint64_t shots,
-                                                uint64_t *out) {
+__global__ __launch_bounds__(TPB) void k_sample_walk(const typename VecOf<T>::type *a, uint64_t N, uint64_t bl,
+                                                     uint64_t nblk, const double *hi, const double *total,
+                                                     const double *us, uint64_t shots, uint64_t *out) {
+#pragma clang fp contract(off)
     using V = typename VecOf<T>::type;
     for (uint64_t s = gtid(); s < shots; s += gstride()) {
-    const double u = us[s] * *total;
-    uint64_t lo = 0, hi = nblk;  // first block with u < excl + w
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (u < excl[mid] + w[mid])
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    const uint64_t b = lo < nblk ? lo : nblk - 1;
-    double run = excl[b];
-    const uint64_t start = b * bl, end = min(start + bl, N);
-    uint64_t idx = start;
-    while (idx + 1 < end) {
-        const V v = a[idx];
-        const double p = (double)v.x * v.x + (double)v.y * v.y;
-        if (run + p > u) break;
-        run += p;
-        idx++;
-    }
-    uint64_t pick = idx;
-    while (pick > start && czero(a[pick])) pick--;
-    out[s] = pick;
+        const double u = us[s] * *total;
+        uint64_t lo = 0, up = nblk;
+        while (lo < up) {
+            const uint64_t mid = (lo + up) >> 1;
+            if (u < hi[mid])
+                up = mid;
+            else
+                lo = mid + 1;
+        }
+        const uint64_t b = lo < nblk ? lo : nblk - 1;
+        double run = b ? hi[b - 1] : 0.0;
+        const uint64_t start = b * bl, end = min(start + bl, N);
+        uint64_t idx = start;
+        while (idx + 1 < end) {
+            const double p = abs_sq_f64(a[idx]);
+            if (!(run + p <= u)) break;
+            run += p;
+            idx++;
+        }
+        uint64_t pick = idx;
+        while (pick > start && czero(a[pick])) pick--;
+        out[s] = pick;
     }
 }
 
@@ -1057,49 +1085,78 @@ void kron(ncsv_state *a, ncsv_state *b, ncsv_state *out) {
     sync(out);
 }
 
+// Scratch of one sampling call: block prefixes | total (2 doubles) | draws |
+// picks.
+struct SampleBufs {
+    double *hi, *total, *us;
+    uint64_t *out;
+};
+
+static size_t sample_doubles(uint64_t nblk, uint64_t shots) { return nblk + 2 + 2 * shots; }
+
+static SampleBufs sample_carve(void *base, uint64_t nblk, uint64_t shots) {
+    SampleBufs b;
+    b.hi = (double *)base;
+    b.total = b.hi + nblk;
+    b.us = b.total + 2;
+    b.out = (uint64_t *)(b.us + shots);
+    return b;
+}
+
+// The sampling launches over caller-provided device scratch `base` (at
+// least sample_doubles() doubles): draws up, block weights, fold, ZeroState
+// check, walk, picks down.
 template <typename T>
-static void sample_t(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out) {
+static void sample_on(ncsv_state *s, void *base, uint64_t shots, const double *uniforms, uint64_t *out) {
     using V = typename VecOf<T>::type;
     const uint64_t N = 1ull << s->n;
     const uint64_t bl = std::min<uint64_t>(SAMPLE_BLOCK, N);
     const uint64_t nblk = N / bl;
-    // weights | excl | total | uniforms | out
-    const size_t need = 2 * nblk + 2 + 2 * shots;
-    double *buf = nullptr;
-    check_hip(hipMalloc(&buf, need * sizeof(double)), "hipMalloc(sample)");
-    double *w = buf, *excl = buf + nblk, *tot = buf + 2 * nblk, *us = buf + 2 * nblk + 2;
-    uint64_t *dout = (uint64_t *)(us + shots);
-    check_hip(hipMemcpyAsync(us, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, s->stream), "uniforms");
+    const uint32_t C = (uint32_t)std::min<uint64_t>(bl, 128 / sizeof(V));  // 128-byte runs
+    const SampleBufs sb = sample_carve(base, nblk, shots);
+    if (shots)
+        check_hip(hipMemcpyAsync(sb.us, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, s->stream),
+                  "upload draws");
     {
         KTimer kt(s, "sample_weights", (double)s->amp_bytes());
-        hipLaunchKernelGGL(k_block_weights<T>, dim3((uint32_t)((nblk * 64 + TPB - 1) / TPB)), dim3(TPB), 0, s->stream,
-                           (const V *)s->amp, N, bl, nblk, w);
+        hipLaunchKernelGGL(k_block_weights<T>, dim3((uint32_t)((nblk + 63) / 64)), dim3(64), 0, s->stream,
+                           (const V *)s->amp, bl, nblk, C, sb.hi);
         launch_check("k_block_weights");
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s->stream, (const double *)w, nblk, excl, tot);
-    launch_check("k_scan");
-    check_hip(hipMemcpyAsync(s->h_part, tot, sizeof(double), hipMemcpyDeviceToHost, s->stream), "total");
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(1), 0, s->stream, sb.hi, nblk, sb.total);
+    launch_check("k_fold");
+    check_hip(hipMemcpyAsync(s->h_part, sb.total, sizeof(double), hipMemcpyDeviceToHost, s->stream), "total");
     sync(s);
-    if (!(s->h_part[0] > 0.0)) {
-        (void)hipFree(buf);
-        fail(NCSV_ZERO_STATE, "state vanished: sample");
-    }
+    if (!(s->h_part[0] > 0.0)) fail(NCSV_ZERO_STATE, "state vanished: sample");
     if (shots) {
-        hipLaunchKernelGGL(k_sample<T>, dim3(nblocks(shots)), dim3(TPB), 0, s->stream, (const V *)s->amp, N, bl, nblk,
-                           (const double *)w, (const double *)excl, (const double *)tot, (const double *)us, shots,
-                           dout);
-        launch_check("k_sample");
-        check_hip(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream), "shots");
+        hipLaunchKernelGGL(k_sample_walk<T>, dim3(nblocks(shots)), dim3(TPB), 0, s->stream, (const V *)s->amp, N, bl,
+                           nblk, (const double *)sb.hi, (const double *)sb.total, (const double *)sb.us, shots,
+                           sb.out);
+        launch_check("k_sample_walk");
+        check_hip(hipMemcpyAsync(out, sb.out, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream), "shots");
     }
     sync(s);
-    (void)hipFree(buf);
+}
+
+static uint64_t sample_nblk(const ncsv_state *s) {
+    const uint64_t N = 1ull << s->n;
+    return N / std::min<uint64_t>(SAMPLE_BLOCK, N);
 }
 
 void sample(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out) {
+    // per-handle scratch, grown on demand and kept with the state
+    const size_t bytes = sample_doubles(sample_nblk(s), shots) * sizeof(double);
+    if (s->samp_cap < bytes) {
+        if (s->d_samp) (void)hipFree(s->d_samp);
+        s->d_samp = nullptr;
+        s->samp_cap = 0;
+        check_hip(hipMalloc(&s->d_samp, bytes), "hipMalloc(sampling scratch)");
+        s->samp_cap = bytes;
+    }
     if (s->prec == NCSV_C128)
-        sample_t<double>(s, shots, uniforms, out);
+        sample_on<double>(s, s->d_samp, shots, uniforms, out);
     else
-        sample_t<float>(s, shots, uniforms, out);
+        sample_on<float>(s, s->d_samp, shots, uniforms, out);
 }
 
 }  // namespace dev

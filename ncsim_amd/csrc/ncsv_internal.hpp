@@ -138,6 +138,9 @@ struct ncsv_state {
     size_t part_cap = 0;
     void *d_tab = nullptr;
     size_t tab_cap = 0;
+    // sampling scratch (block prefixes, draws, picks), grown on demand
+    void *d_samp = nullptr;
+    size_t samp_cap = 0;
     double *h_part = nullptr;  // pinned
     size_t h_part_cap = 0;
     int fusion = 1;

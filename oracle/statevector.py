@@ -263,7 +263,12 @@ class StateVector:
             raise SimError("Invalid", f"measured operator {p!r} is not Hermitian")
         e = self.expectation(p).real
         nn = self.norm_sq().real
-        p_plus = min(max((1.0 + e / nn) / 2.0, 0.0), 1.0)
+        # IEEE division (0/0 = NaN, e/0 = +-inf) and f64::clamp, which keeps a
+        # NaN: a zero state draws one uniform and takes outcome 0 (mod.rs:122-128)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            p_plus = float((1.0 + np.float64(e) / np.float64(nn)) / 2.0)
+        if not math.isnan(p_plus):
+            p_plus = min(max(p_plus, 0.0), 1.0)
         if p_plus > 1.0 - 1e-12:
             outcome = False
         elif p_plus < 1e-12:

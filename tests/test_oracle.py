@@ -479,3 +479,113 @@ def test_errors_follow_the_reference():
     with pytest.raises(SimError) as e:
         w.normalize()
     assert e.value.kind == "ZeroState"
+
+
+# ------------------------------------------- inner, expectation_sum, probability
+def dense_pauli(n, x, z, phase):
+    """i^phase X^x Z^z as a 2^n x 2^n matrix built from 2x2 factors (qubit q
+    is bit q of the index, so qubit 0 is the last Kronecker factor)."""
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Z = np.array([[1, 0], [0, -1]], dtype=complex)
+    m = np.ones((1, 1), dtype=complex)
+    for q in reversed(range(n)):
+        f = np.eye(2, dtype=complex)
+        if (x >> q) & 1:
+            f = f @ X
+        if (z >> q) & 1:
+            f = f @ Z
+        m = np.kron(m, f)
+    return (1j ** phase) * m
+
+
+def unnormalized_state(n, seed, scale):
+    rng = np.random.default_rng(seed)
+    a = (rng.random(1 << n) - 0.5) + 1j * (rng.random(1 << n) - 0.5)
+    return scale * a
+
+
+def test_inner_is_the_conjugate_linear_dot_product():
+    """vector.rs:431-443: <a|b> = sum conj(a_i) b_i, conjugating the left
+    state; inner(a, a) = norm_sq; a size mismatch is Invalid (432-434)."""
+    for n, seed in [(1, 1), (5, 2), (11, 3), (15, 4)]:
+        a = unnormalized_state(n, seed, 1.7)
+        b = unnormalized_state(n, seed + 100, 0.3) + 0.2j
+        sa, sb = StateVector(n, a.copy()), StateVector(n, b.copy())
+        want = np.vdot(a, b)
+        got = sa.inner(sb)
+        assert abs(got - want) <= 1e-12 * max(1.0, abs(want))
+        # the left argument is conjugated: swapping gives the conjugate
+        assert abs(sb.inner(sa) - np.conj(want)) <= 1e-12 * max(1.0, abs(want))
+        # a conjugated right argument would give sum a_i b_i... which differs here
+        assert abs(np.sum(np.conj(a) * np.conj(b)) - want) > 1e-3
+        nn = sa.norm_sq()
+        assert abs(sa.inner(sa) - nn) <= 1e-12 * nn.real
+        assert abs(nn - np.vdot(a, a).real) <= 1e-12 * nn.real and nn.imag == 0.0
+    with pytest.raises(SimError) as e:
+        StateVector.zero_state(3).inner(StateVector.zero_state(4))
+    assert e.value.kind == "Invalid"
+
+
+def test_expectation_sum_is_the_dense_quadratic_form():
+    """mod.rs:93-101: sum_k c_k <psi|P_k|psi> with complex coefficients and
+    Paulis of every phase (non-Hermitian terms included), against psi^H H psi
+    with H built densely from 2x2 factors.  The state is unnormalized, so the
+    result is not divided by the norm."""
+    rng = np.random.default_rng(7)
+    for n in (3, 6, 10):
+        a = unnormalized_state(n, n, 2.5)
+        s = StateVector(n, a.copy())
+        terms = []
+        H = np.zeros((1 << n, 1 << n), dtype=complex)
+        for _ in range(11):
+            x, z, ph = int(rng.integers(0, 1 << n)), int(rng.integers(0, 1 << n)), int(rng.integers(0, 4))
+            c = complex(rng.normal(), rng.normal())
+            terms.append((c, Pauli(n, x, z, ph)))
+            H += c * dense_pauli(n, x, z, ph)
+        want = np.vdot(a, H @ a)
+        got = s.expectation_sum(terms)
+        assert abs(got - want) <= 1e-11 * max(1.0, abs(want)), (n, got, want)
+        # a conjugated coefficient gives something else
+        wrong = sum(np.conj(c) * np.vdot(a, dense_pauli(n, p.x, p.z, p.phase) @ a) for c, p in terms)
+        assert abs(wrong - want) > 1e-3
+        # each term alone is the dense form too (expectation, vector.rs:412-429)
+        for c, p in terms[:4]:
+            e = s.expectation(p)
+            assert abs(e - np.vdot(a, dense_pauli(n, p.x, p.z, p.phase) @ a)) <= 1e-11 * np.vdot(a, a).real
+
+
+def test_probability_divides_by_the_norm():
+    """mod.rs:104-108: |<x|psi>|^2 / <psi|psi> on unnormalized states."""
+    for n, scale in [(4, 3.0), (9, 0.01), (13, 17.0)]:
+        a = unnormalized_state(n, 30 + n, scale)
+        s = StateVector(n, a.copy())
+        norm = float(np.vdot(a, a).real)
+        assert abs(norm - 1.0) > 1e-3
+        ps = [s.probability(x) for x in range(0, 1 << n, max(1, (1 << n) // 37))]
+        want = [abs(a[x]) ** 2 / norm for x in range(0, 1 << n, max(1, (1 << n) // 37))]
+        assert np.allclose(ps, want, rtol=1e-13, atol=0)
+    # over every index the probabilities sum to one
+    a = unnormalized_state(6, 5, 4.0)
+    s = StateVector(6, a.copy())
+    assert abs(sum(s.probability(x) for x in range(64)) - 1.0) < 1e-13
+
+
+def test_measuring_a_zero_state_draws_and_takes_outcome_zero():
+    """mod.rs:120-128: norm 0 gives p_plus = NaN, which f64::clamp keeps, so
+    neither certainty branch is taken, one uniform is drawn and `u >= NaN`
+    is false: outcome false, weight 0."""
+
+    class Draws:
+        def __init__(self):
+            self.count = 0
+
+        def gen_f64(self):
+            self.count += 1
+            return 0.75
+
+    s = StateVector.zero_state(3)
+    assert s.project(Pauli.parse("Z0", 3), True) == 0
+    d = Draws()
+    assert s.measure(Pauli.parse("Z1", 3), d) is False
+    assert d.count == 1
+    assert s.norm_sq() == 0
