@@ -174,6 +174,15 @@ ncsv_status ncsv_normalize(ncsv_state *s);
 /* QuantumState::sample (vector.rs:494-540) with the shots' draws given:
  * uniforms[k] in [0, 1) in draw order; out[k] = basis index of shot k. */
 ncsv_status ncsv_sample(const ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out);
+/* No counterpart in the reference: ncsv_sample for callers that cannot
+ * allocate device memory during the call (serving loops, harnesses that
+ * forbid hipMalloc).  The same picks as ncsv_sample; the scratch is a fixed
+ * 32 MiB array per GPU held by the library until the process ends, so
+ * 2^(n-12) + 2 + 2 x shots must be at most 4,194,304 (Invalid otherwise;
+ * e.g. about 1.9 million shots at 30 qubits).  Calls on one GPU share the
+ * array and are serialized by the library (overlapping calls would race).
+ * A deferred basis state is still written first, as by every reader. */
+ncsv_status ncsv_sample_noalloc(const ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out);
 /* The same with the draws taken from `rng` (shots draws, in order). */
 ncsv_status ncsv_sample_rng(const ncsv_state *s, uint64_t shots, ncsv_uniform_fn rng, void *ctx, uint64_t *out);
 /* QuantumState::tensor / StateVector::kron (vector.rs:168-188, 546-548):

@@ -49,6 +49,7 @@ _SIGS = {
     "ncsv_scale": [ctypes.c_void_p, c_dp],
     "ncsv_normalize": [ctypes.c_void_p],
     "ncsv_sample": [ctypes.c_void_p, ctypes.c_uint64, c_dp, c_u64p],
+    "ncsv_sample_noalloc": [ctypes.c_void_p, ctypes.c_uint64, c_dp, c_u64p],
     "ncsv_sample_rng": [ctypes.c_void_p, ctypes.c_uint64, UNIFORM_FN, ctypes.c_void_p, c_u64p],
     "ncsv_tensor": [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
     "ncsv_circuit_new": [ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],

@@ -582,6 +582,19 @@ ncsv_status ncsv_sample(const ncsv_state *s, uint64_t shots, const double *unifo
     });
 }
 
+ncsv_status ncsv_sample_noalloc(const ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out) {
+    return guard([&] {
+        need(s, "state");
+        if (shots) {
+            need(uniforms, "uniforms");
+            need(out, "out");
+        }
+        std::lock_guard<std::recursive_mutex> lk(s->mu);
+        materialize(const_cast<ncsv_state *>(s));
+        dev::sample_noalloc(const_cast<ncsv_state *>(s), shots, uniforms, out);
+    });
+}
+
 ncsv_status ncsv_sample_rng(const ncsv_state *s, uint64_t shots, ncsv_uniform_fn rng, void *ctx, uint64_t *out) {
     return guard([&] {
         if (!rng) fail(NCSV_INVALID, "rng is null");

@@ -1143,6 +1143,60 @@ static uint64_t sample_nblk(const ncsv_state *s) {
     return N / std::min<uint64_t>(SAMPLE_BLOCK, N);
 }
 
+}  // namespace dev
+}  // namespace ncsv
+
+// Device scratch of ncsv_sample_noalloc: one array per GPU, held until the
+// process ends (32 MiB: block prefixes, the total, draws and picks).  Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the device (the array's name is a host-side stand-in there).
+constexpr size_t NOALLOC_DOUBLES = size_t(4) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace ncsv {
+namespace dev {
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+// every call on one GPU uses the same array: calls are serialized per device
+std::mutex g_scratch_use[MAX_DEVICES];
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current.
+void *scratch_on(int device) {
+    if (device < 0 || device >= MAX_DEVICES) fail(NCSV_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        check_hip(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != device) check_hip(hipSetDevice(device), "hipSetDevice");
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        check_hip(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+void sample_noalloc(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out) {
+    const uint64_t nblk = sample_nblk(s);
+    if (shots > NOALLOC_DOUBLES || sample_doubles(nblk, shots) > NOALLOC_DOUBLES)
+        fail(NCSV_INVALID, "sample_noalloc: " + std::to_string(shots) + " shots on " + std::to_string(s->n) +
+                               " qubits need more than the fixed scratch of " + std::to_string(NOALLOC_DOUBLES) +
+                               " doubles (block count + 2 + 2 x shots)");
+    void *ptr = scratch_on(s->device);
+    // nothing in the array must start at zero: every word read is written
+    // earlier in the same call
+    std::lock_guard<std::mutex> lk(g_scratch_use[s->device]);
+    if (s->prec == NCSV_C128)
+        sample_on<double>(s, ptr, shots, uniforms, out);
+    else
+        sample_on<float>(s, ptr, shots, uniforms, out);
+}
+
 void sample(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out) {
     // per-handle scratch, grown on demand and kept with the state
     const size_t bytes = sample_doubles(sample_nblk(s), shots) * sizeof(double);

@@ -187,6 +187,8 @@ double project(ncsv_state *s, uint64_t x, uint64_t z, uint32_t ph, int outcome);
 void scale(ncsv_state *s, cd c);
 void kron(ncsv_state *a, ncsv_state *b, ncsv_state *out);
 void sample(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out);
+// sample() over the library's fixed per-device scratch array (no allocation).
+void sample_noalloc(ncsv_state *s, uint64_t shots, const double *uniforms, uint64_t *out);
 void sync(const ncsv_state *s);
 }  // namespace dev
 
