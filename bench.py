@@ -5,9 +5,12 @@ pairs) followed by 64 random Pauli expectations (weight 1-4).
 
 One step = reset to |0...0>, run the circuit, compute the 64 expectations.
 amp-updates = (unfused gate count) x 2^n per step.  Prints one JSON line with
-the roofline of the dominant kernel (HIP-event timed on the state's stream)
-and a CPU baseline (the C oracle, a bounded sample of the same circuit family
-on this host's cores).
+the roofline of the dominant kernel (HIP-event timed on the state's stream),
+the precision form measured in the same run (QFT + 2000 RZ: 33 qubits
+complex64, 32 qubits complex128 and complex64, fidelity and max-abs error of
+complex64 against complex128), and a CPU baseline (the C oracle on a bounded
+sample of the 26-qubit member of the brickwork family, on all of this host's
+cores and on one thread).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N]
 """
@@ -38,12 +41,22 @@ def measured_traffic(n, depth, fusion, kernel):
     return t["bytes_per_launch"] if t.get("kernel") == kernel else None
 
 
-def cpu_baseline(seconds=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(seconds=12.0, seconds_one=8.0):
     """The oracle (C restatement of StateVector<C64>, OpenMP) applying the
     26-qubit member of the same circuit family gate by gate until `seconds`
-    have passed."""
-    import numpy as np
-
+    have passed, on every thread OpenMP is given (OMP_NUM_THREADS, else all
+    cores) and again on one thread."""
     from oracle.circuit import Circuit as OCircuit
     from oracle.statevector import StateVector, lib
     from ncsim_amd import workloads as W
@@ -51,77 +64,126 @@ def cpu_baseline(seconds=15.0):
     n = 26
     circ = OCircuit.parse(W.rotation_brickwork(n, 24, 3))
     v = StateVector.zero_state(n)
-    t0 = time.perf_counter()
-    done = 0
-    for op in circ.ops:
-        v.apply_gate(op[1], op[2])
-        done += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
+
+    def timed(limit):
+        t0 = time.perf_counter()
+        done = 0
+        for op in circ.ops:
+            v.apply_gate(op[1], op[2])
+            done += 1
+            if time.perf_counter() - t0 > limit:
+                break
+        return done, time.perf_counter() - t0
+
     threads = lib().orc_num_threads()
+    done, dt = timed(seconds)
+    lib().orc_set_num_threads(1)
+    done1, dt1 = timed(seconds_one)
+    lib().orc_set_num_threads(threads)
     del v
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
     return {"value": done * float(1 << n) / dt, "unit": "amp-updates/s", "cores": threads, "kind": "port",
             "sample": f"first {done} gates of the 26-qubit depth-24 rotation brickwork, "
-                      f"{dt:.1f} s, {threads} OpenMP threads"}
+                      f"{dt:.1f} s, {threads} OpenMP threads",
+            "single_thread": {"value": done1 * float(1 << n) / dt1, "cores": 1,
+                              "sample": f"first {done1} gates of the same circuit, {dt1:.1f} s, 1 thread"},
+            "host": {"nproc": os.cpu_count(), "affinity": affinity, "cpu_model": cpu_model(),
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
+            "what": "C restatement of ncsim StateVector<C64> (oracle/sv_oracle.c), -O3, OpenMP"}
 
 
-def run_qft(args):
-    """Precision form: QFT (H + CPHASE(1, 2..n)) with 2000 RZ, a seeded X
-    prefix.  complex128 and complex64 at n-1 qubits (fidelity and max-abs
-    error of complex64 against complex128, on the device), then complex64 at
-    n qubits.  value = complex64 amp-updates/s at n qubits."""
+def precision_form(n=33, steps=3, fusion=1, only_n=False):
+    """The precision form on QFT (H + CPHASE(1, 2..n)) with 2000 RZ after a
+    seeded X layer: complex128 and complex64 at n-1 qubits (fidelity and
+    max-abs error of complex64 against complex128, on the device, and the
+    complex64 / complex128 speed-up at equal size), then complex64 at n
+    qubits with its dominant kernel's roofline.  One step = set_basis(0) +
+    the circuit; `steps` timed after one untimed step (plans, compiles)."""
     import ncsim_amd as H
     from ncsim_amd import workloads as W
-
-    n = args.qubits if args.qubits != 30 else 33
-    res = {}
 
     def timed(m, prec):
         text = W.qft_with_rz(m, 2000, 7)
         gates = W.count_gates(text)
         circ = H.Circuit.parse(text)
         s = H.HipStateVector.zero_state(m, prec)
-        s.set_fusion(args.fusion)
+        s.set_fusion(fusion)
         s.set_basis(0)
         s.run(circ)  # warm-up (plans, kernels), the same calls as a timed step
         s.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             s.set_basis(0)
             s.run(circ)
         s.synchronize()
-        ms = (time.perf_counter() - t0) * 1000 / args.steps
+        ms = (time.perf_counter() - t0) * 1000 / steps
         s.set_basis(0)
         s.profile_reset()
         s.profile(True)
         s.run(circ)
         s.profile(False)
         st = s.profile_read()
-        log(f"qft: n={m} prec={prec} gates={gates} {ms:.1f} ms  " +
+        sweeps = s.sweep_count()
+        log(f"qft: n={m} prec={'c128' if prec == H.C128 else 'c64'} gates={gates} {ms:.1f} ms/step  " +
             ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(st.items())))
-        return s, ms, gates, st
+        return s, ms, gates, st, sweeps
 
-    a, ms128, g1, st128 = timed(n - 1, H.C128)
-    b, ms64s, _, _ = timed(n - 1, H.C64)
+    if only_n:  # the n-qubit complex64 part alone (counter runs)
+        c, ms64, g2, st64, sweeps = timed(n, H.C64)
+        return None
+    a, ms128, g1, _, _ = timed(n - 1, H.C128)
+    b, ms64s, _, _, _ = timed(n - 1, H.C64)
     cmp = a.compare(b)
     del a, b
-    c, ms64, g2, st64 = timed(n, H.C64)
+    c, ms64, g2, st64, sweeps = timed(n, H.C64)
     del c
     name, (launches, tot_ms, byts) = max(st64.items(), key=lambda kv: kv[1][1])
     achieved = byts / launches / (tot_ms / launches / 1000.0)
-    out = {"metric": "amp-updates/s", "value": g2 * float(1 << n) / (ms64 / 1000.0), "unit": "amp-updates/s",
-           "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms64, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32, f64 reductions)",
-           "data": "synthetic (seeded circuit)",
-           "config": {"workload": f"QFT + 2000 RZ, {n} qubits complex64; {n - 1} qubits complex128 vs complex64",
-                      "gates": g2},
-           "precision_check": {"qubits": n - 1, "fidelity": cmp["fidelity"], "fidelity_loss": 1 - cmp["fidelity"],
-                               "max_abs": cmp["max_abs"], "ms_c128": ms128, "ms_c64": ms64s,
-                               "speedup_c64_over_c128": ms128 / ms64s},
-           "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None, "launches": launches,
-                        "avg_ms": tot_ms / launches}}
+    return {"workload": f"QFT + 2000 RZ (seeded X prefix): {n} qubits complex64; {n - 1} qubits complex128 "
+                        f"and complex64", "qubits": n, "gates": g2, "steps": steps,
+            "ms_per_step": ms64, "value": g2 * float(1 << n) / (ms64 / 1000.0), "unit": "amp-updates/s",
+            "dtype": "c64 (f32, f64 reductions)",
+            "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                         "traffic": measured_traffic_c2(n, name),
+                         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                         "algorithmic_bytes": byts / launches, "launches": launches, "avg_ms": tot_ms / launches,
+                         "sweeps_per_step": sweeps},
+            "check": {"qubits": n - 1, "fidelity": cmp["fidelity"], "fidelity_loss": 1 - cmp["fidelity"],
+                      "max_abs": cmp["max_abs"]},
+            "c128": {"qubits": n - 1, "ms_per_step": ms128, "value": g1 * float(1 << (n - 1)) / (ms128 / 1000.0)},
+            "c64_same_size": {"qubits": n - 1, "ms_per_step": ms64s,
+                              "value": g1 * float(1 << (n - 1)) / (ms64s / 1000.0)},
+            "speedup_c64_over_c128": ms128 / ms64s,
+            "gate_layers_per_s_ratio_n_c64_vs_n1_c128": (ms128 / ms64)}
+
+
+def measured_traffic_c2(n, kernel):
+    """HBM bytes per launch of the complex64 QFT's dominant kernel from the
+    committed counter runs (profiles/traffic_sweep_c64_qft33.json), else None."""
+    path = os.path.join(ROOT, "profiles", "traffic_sweep_c64_qft33.json")
+    if n != 33 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["bytes_per_launch"] if t.get("kernel") == kernel else None
+
+
+def run_qft(args):
+    """The precision form alone; value = complex64 amp-updates/s at n qubits."""
+    n = args.qubits if args.qubits != 30 else 33
+    pf = precision_form(n, args.steps, args.fusion, args.only_n)
+    if pf is None:
+        return
+    out = {"metric": "amp-updates/s", "value": pf["value"], "unit": "amp-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 1, "ms_per_step": pf["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": pf["dtype"], "data": "synthetic (seeded circuit)",
+           "config": {"workload": pf["workload"], "gates": pf["gates"]}, "roofline": pf["roofline"],
+           "precision_check": pf["check"], "c128": pf["c128"], "c64_same_size": pf["c64_same_size"],
+           "speedup_c64_over_c128": pf["speedup_c64_over_c128"]}
     print(json.dumps(out), flush=True)
 
 
@@ -352,6 +414,8 @@ def main():
     ap.add_argument("--depth", type=int, default=24)
     ap.add_argument("--fusion", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true")
+    ap.add_argument("--only-n", action="store_true", help="qft workload: only the n-qubit complex64 runs")
     ap.add_argument("--workload", default="brickwork", choices=["brickwork", "qft", "batch", "trajectories"])
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--stride", type=int, default=8)
@@ -427,6 +491,10 @@ def main():
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                 "algorithmic_bytes": byts / launches, "launches": launches,
                 "avg_ms": tot_ms / launches, "sweeps_per_step": sweeps}
+    del s
+    pf = None
+    if not args.no_precision_form and n == 30:
+        pf = precision_form(33, max(1, min(args.steps, 3)), args.fusion)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "amp-updates/s", "value": value, "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -435,7 +503,7 @@ def main():
         "config": {"workload": f"rotation brickwork {n} qubits depth {depth} + 64 Pauli expectations",
                    "qubits": n, "depth": depth, "gates": gates, "fusion": args.fusion,
                    "precision": "complex128" if prec == H.C128 else "complex64"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "precision_form": pf, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -39,6 +39,9 @@ struct Gen {
     bool dbl = true;
     double scale = 1.0;  // common factor taken out of the real-form 2x2s, applied at the store
     bool opaque = false;  // exchange addresses recomputed where used (kernels that loop over tiles)
+    int S = 4;            // LDS swizzle width (16-byte slots: 4; 8-byte: 5)
+    int half = 0;         // exchanges through half a tile of LDS, in two passes (2, 3: test the
+                          // wavefront-turn and intermediate-layout forms wherever they apply)
 
     uint32_t push(double v) {
         coef->push_back(v);
@@ -62,6 +65,21 @@ struct Gen {
 bool use_literals() {
     const char *e = getenv("NCSV_JIT_LITERALS");
     return !(e && atoi(e) == 0);
+}
+
+// Sweep kernel shape knobs: exchanges go through half a tile of LDS in two
+// passes by default (NCSV_JIT_HALF=0: the whole tile at once), so three or
+// four workgroups share a CU instead of two (30-qubit rotation brickwork:
+// complex64 sweeps 4.41 -> 3.89 ms, complex128 7.28 -> 7.15 ms);
+// NCSV_JIT_WAVES=w asks the compiler for w wavefronts per SIMD (registers
+// capped at 512 / w).
+int use_half() {
+    const char *e = getenv("NCSV_JIT_HALF");
+    return e ? atoi(e) : 1;
+}
+int jit_waves() {
+    const char *e = getenv("NCSV_JIT_WAVES");
+    return e ? std::max(0, std::min(8, atoi(e))) : 0;
 }
 
 // Expression depositing bits 0..k-1 of `src` on the bit positions `dst`
@@ -89,6 +107,32 @@ std::vector<uint32_t> positions(uint32_t mask, uint32_t L, bool set) {
     for (uint32_t p = 0; p < L; p++)
         if ((((mask >> p) & 1) != 0) == set) v.push_back(p);
     return v;
+}
+
+// LDS slot of tile index (thread part | off) under the XOR swizzle of the
+// kernels, swz(t) = t ^ (((t >> S) ^ (t >> 2S)) & (2^S - 1)).  swz is linear
+// over XOR and the two parts have disjoint bits, so the slot is
+// swz(thread part) ^ swz(off): with `sw` = swz(thread part) in a register
+// (bits only on `tbits` and the low S bits), the constant's other bits can
+// be added (an immediate offset of the LDS instruction) and only the rest is
+// XORed.
+uint32_t swz_host(uint32_t t, int S) { return t ^ (((t >> S) ^ (t >> (2 * S))) & ((1u << S) - 1)); }
+std::string lidx(int S, const char *sw, uint32_t tbits, uint32_t off) {
+    const uint32_t c = swz_host(off, S);
+    const uint32_t live = tbits | ((1u << S) - 1);
+    const uint32_t lo = c & live, hi = c & ~live;
+    std::ostringstream e;
+    if (lo)
+        e << "(" << sw << " ^ " << lo << "u)";
+    else
+        e << sw;
+    if (hi) e << " + " << hi << "u";
+    return e.str();
+}
+uint32_t bits_of(const std::vector<uint32_t> &pos) {
+    uint32_t m = 0;
+    for (uint32_t q : pos) m |= 1u << q;
+    return m;
 }
 
 // i^k * (variable) as an expression pair
@@ -134,25 +178,166 @@ std::string cond_expr(const Gen &g, uint64_t gmask, uint32_t cur) {
     return e.str();
 }
 
+// The exchange between register layouts `from` and `to` through half a tile
+// of LDS, for layouts sharing a register position p: the amplitudes whose
+// tile position p holds h go in pass h, each thread writing the half of its
+// registers with that bit and reading the half of the new layout with it.
+// Each pass reads into the variables its own writes just freed (the
+// generator renames registers), so no value is overwritten before it is
+// stored and no extra registers are needed.  Layouts with no common
+// register position go through an intermediate layout sharing one with
+// each.  LDS use halves at the price of two more barriers per exchange.
+// A tile position that is the same wavefront-index bit of the thread
+// index in both layouts: whole wavefronts write and then read in the pass
+// of their bit, so none overwrites a register it has not stored.  -1 if none.
+int common_wave_bit(const Gen &g, uint32_t from, uint32_t to) {
+    const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
+    for (int i = (int)g.LT - 1; i >= 6; i--)
+        if (ta[i] == tb[i]) return i;
+    return -1;
+}
+
+void emit_exchange_wave_halves(Gen &g, uint32_t from, uint32_t to, int i) {
+    auto &o = g.o;
+    const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
+    const std::vector<uint32_t> ra = positions(from, g.L, true), rb = positions(to, g.L, true);
+    const uint32_t p = ta[i];
+    auto squeeze = [&](uint32_t q) { return q < p ? q : q - 1; };
+    auto tpart = [&](const std::vector<uint32_t> &tpos) {
+        std::vector<uint32_t> d;
+        for (size_t k = 0; k < tpos.size(); k++)
+            if ((int)k != i) d.push_back(squeeze(tpos[k]));
+        // the thread index without bit i
+        return deposit("((tid & " + std::to_string((1u << i) - 1) + "u) | ((tid >> " + std::to_string(i + 1) +
+                           "u) << " + std::to_string(i) + "u))",
+                       d, false);
+    };
+    auto roffs = [&](const std::vector<uint32_t> &rpos, uint32_t r) {
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if ((r >> j) & 1) off |= 1u << squeeze(rpos[j]);
+        return off;
+    };
+    o << "  { // exchange in halves on tile position " << p << " (wavefronts take turns)\n";
+    o << "    unsigned ta = " << tpart(ta) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(ta));\n";
+    o << "    unsigned tb = " << tpart(tb) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(tb));\n";
+    o << "    const unsigned sa = swz(ta), sb = swz(tb);\n";
+    std::vector<uint32_t> sqa, sqb;
+    for (size_t k = 0; k < ta.size(); k++)
+        if ((int)k != i) {
+            sqa.push_back(squeeze(ta[k]));
+            sqb.push_back(squeeze(tb[k]));
+        }
+    const uint32_t ma = bits_of(sqa), mb = bits_of(sqb);
+    for (uint32_t h = 0; h < 2; h++) {
+        o << "    if (((tid >> " << i << ") & 1u) == " << h << "u) {\n";
+        for (uint32_t r = 0; r < g.NR; r++)
+            o << "      lds[" << lidx(g.S, "sa", ma, roffs(ra, r)) << "] = " << g.var(r) << ";\n";
+        o << "    }\n    __syncthreads();\n";
+        o << "    if (((tid >> " << i << ") & 1u) == " << h << "u) {\n";
+        for (uint32_t r = 0; r < g.NR; r++)
+            o << "      " << g.var(r) << " = lds[" << lidx(g.S, "sb", mb, roffs(rb, r)) << "];\n";
+        o << "    }\n    __syncthreads();\n";
+    }
+    o << "  }\n";
+}
+
+void emit_exchange_half(Gen &g, uint32_t from, uint32_t to) {
+    if (!(from & to) || (g.half >= 2 && from != to)) {
+        const int i = g.half == 3 ? -1 : common_wave_bit(g, from, to);
+        if (i >= 0) {
+            emit_exchange_wave_halves(g, from, to, i);
+            return;
+        }
+        if (!(from & to)) {
+            const uint32_t hi = 31u - (uint32_t)__builtin_clz(from);
+            const uint32_t mid = (from & ~(1u << hi)) | (1u << (uint32_t)__builtin_ctz(to));
+            const int keep = g.half;
+            g.half = 1;
+            emit_exchange_half(g, from, mid);
+            emit_exchange_half(g, mid, to);
+            g.half = keep;
+            return;
+        }
+    }
+    auto &o = g.o;
+    const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
+    const std::vector<uint32_t> ra = positions(from, g.L, true), rb = positions(to, g.L, true);
+    const uint32_t p = (uint32_t)__builtin_ctz(from & to);
+    uint32_t ja = 0, jb = 0;
+    for (uint32_t j = 0; j < g.R; j++) {
+        if (ra[j] == p) ja = j;
+        if (rb[j] == p) jb = j;
+    }
+    auto squeeze = [&](uint32_t q) { return q < p ? q : q - 1; };
+    auto tpart = [&](const std::vector<uint32_t> &tpos) {
+        std::vector<uint32_t> d;
+        for (uint32_t q : tpos) d.push_back(squeeze(q));
+        return deposit("tid", d, false);
+    };
+    auto roffs = [&](const std::vector<uint32_t> &rpos, uint32_t jp, uint32_t r) {
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < g.R; j++)
+            if (j != jp && ((r >> j) & 1)) off |= 1u << squeeze(rpos[j]);
+        return off;
+    };
+    std::vector<int> renamed(g.NR, -1);
+    o << "  { // exchange in halves on tile position " << p << "\n";
+    o << "    unsigned ta = " << tpart(ta) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(ta));\n";
+    o << "    unsigned tb = " << tpart(tb) << ";\n";
+    if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(tb));\n";
+    o << "    const unsigned sa = swz(ta), sb = swz(tb);\n";
+    uint32_t ma = 0, mb = 0;
+    for (uint32_t q : ta) ma |= 1u << squeeze(q);
+    for (uint32_t q : tb) mb |= 1u << squeeze(q);
+    for (uint32_t h = 0; h < 2; h++) {
+        std::vector<int> freed;
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if (((r >> ja) & 1u) != h) continue;
+            o << "    lds[" << lidx(g.S, "sa", ma, roffs(ra, ja, r)) << "] = " << g.var(r) << ";\n";
+            freed.push_back(g.name[r]);
+        }
+        o << "    __syncthreads();\n";
+        size_t k = 0;
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if (((r >> jb) & 1u) != h) continue;
+            renamed[r] = freed[k++];
+            o << "    a" << renamed[r] << " = lds[" << lidx(g.S, "sb", mb, roffs(rb, jb, r)) << "];\n";
+        }
+        o << "    __syncthreads();\n";
+    }
+    o << "  }\n";
+    g.name = renamed;
+}
+
 void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
+    if (g.half) {
+        emit_exchange_half(g, from, to);
+        return;
+    }
     auto &o = g.o;
     const std::vector<uint32_t> ta = positions(from, g.L, false), tb = positions(to, g.L, false);
     const std::vector<uint32_t> ra = positions(from, g.L, true), rb = positions(to, g.L, true);
     o << "  {\n    unsigned ta = " << deposit("tid", ta, false) << ";\n";
     if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(ta));\n";
+    o << "    const unsigned sa = swz(ta);\n";
     for (uint32_t r = 0; r < g.NR; r++) {
         uint32_t off = 0;
         for (uint32_t j = 0; j < g.R; j++)
             if ((r >> j) & 1) off |= 1u << ra[j];
-        o << "    lds[swz(ta | " << off << "u)] = " << g.var(r) << ";\n";
+        o << "    lds[" << lidx(g.S, "sa", bits_of(ta), off) << "] = " << g.var(r) << ";\n";
     }
     o << "    __syncthreads();\n    unsigned tb = " << deposit("tid", tb, false) << ";\n";
     if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(tb));\n";
+    o << "    const unsigned sb = swz(tb);\n";
     for (uint32_t r = 0; r < g.NR; r++) {
         uint32_t off = 0;
         for (uint32_t j = 0; j < g.R; j++)
             if ((r >> j) & 1) off |= 1u << rb[j];
-        o << "    " << g.var(r) << " = lds[swz(tb | " << off << "u)];\n";
+        o << "    " << g.var(r) << " = lds[" << lidx(g.S, "sb", bits_of(tb), off) << "];\n";
     }
     o << "    __syncthreads();\n  }\n";
 }
@@ -443,6 +628,286 @@ bool diag_run_pays(const std::vector<const TileOpH *> &run, uint32_t NR) {
     return folded < single;
 }
 
+// Diagonal ops applied lazily: a diagonal phase on the registers holding
+// mask m (with conditions on thread or outer bits) commutes with every op
+// that does not act non-diagonally on a register bit of m, and with every
+// other diagonal op.  So each op only multiplies a pending factor for its
+// mask (a compile-time constant, times a per-thread value when conditioned),
+// and the factors are applied to the amplitudes when an op is about to act
+// on one of their bits, before an LDS exchange, or at the store.  Factors on
+// the same mask from many ops (a QFT's controlled phases between two
+// Hadamards) then cost one complex product per amplitude; factors that
+// multiply to 1, -1 or +-i cost nothing or an exact sign or swap.
+bool use_lazy_diagonals() {
+    const char *e = getenv("NCSV_JIT_LAZY");
+    return !(e && atoi(e) == 0);
+}
+
+struct Pending {
+    cd k = cd(1.0, 0.0);  // compile-time part
+    int rt = -1;          // per-thread part: variable D<rt>, or none
+    int sg = -1;          // per-thread sign (conditioned -1s): sign-bit mask S<sg>, or none
+    std::vector<std::pair<uint64_t, cd>> conds;  // conditioned phases (condition bits, phase)
+};
+
+void emit_cmul_var(std::ostringstream &o, const std::string &v, const std::string &re, const std::string &im,
+                   const char *pre) {
+    o << pre << "{ const T pr = " << v << ".x, pi = " << v << ".y; " << v << ".x = pr*" << re << " - pi*" << im << "; "
+      << v << ".y = pr*" << im << " + pi*" << re << "; }\n";
+}
+
+// The thread-index bits (tmask) and block-index bits (bmask: bit i of b is
+// the outer qubit oq[i]) a condition on global index bits tests.
+void split_cond(const Gen &g, uint64_t gmask, uint32_t cur, uint32_t *tmask, uint64_t *bmask) {
+    const std::vector<uint32_t> tp = positions(cur, g.L, false);
+    *tmask = 0;
+    *bmask = 0;
+    for (uint64_t a = gmask; a; a &= a - 1) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(a);
+        int pos = -1;
+        for (uint32_t p = 0; p < g.L; p++)
+            if (g.tq[p] == q) pos = (int)p;
+        if (pos >= 0) {
+            for (size_t i = 0; i < tp.size(); i++)
+                if (tp[i] == (uint32_t)pos) *tmask |= 1u << i;
+        } else {
+            for (size_t i = 0; i < g.oq.size(); i++)
+                if (g.oq[i] == q) *bmask |= 1ull << i;
+        }
+    }
+}
+
+// The per-thread product of conditioned phases as V D<id>: conditions on
+// thread-index bits only go into one table over the thread index (a vector
+// load), conditions within one group of 8 block-index bits into a table
+// over that group (a load at a workgroup-uniform address), and the rest are
+// conditioned products.  Tables sit in the coefficient buffer and are used
+// where they replace two or more products.
+std::string build_runtime(Gen &g, const std::vector<std::pair<uint64_t, cd>> &conds, uint32_t cur, int id) {
+    auto &o = g.o;
+    const std::string d = "D" + std::to_string(id);
+    std::vector<std::pair<uint32_t, cd>> thr;                       // (thread-index mask, phase)
+    std::map<uint32_t, std::vector<std::pair<uint32_t, cd>>> grp;  // group -> (mask in the group, phase)
+    std::vector<std::pair<uint64_t, cd>> rest;                      // conditioned products
+    std::vector<std::pair<uint64_t, cd>> thr_src;
+    std::map<uint32_t, std::vector<std::pair<uint64_t, cd>>> grp_src;
+    for (auto &kv : conds) {
+        uint32_t tm;
+        uint64_t bm;
+        split_cond(g, kv.first, cur, &tm, &bm);
+        if (!bm) {
+            thr.push_back({tm, kv.second});
+            thr_src.push_back(kv);
+            continue;
+        }
+        const uint32_t gi = (uint32_t)__builtin_ctzll(bm) / 8;
+        if (!tm && (bm >> (8 * gi)) < 256) {
+            grp[gi].push_back({(uint32_t)(bm >> (8 * gi)), kv.second});
+            grp_src[gi].push_back(kv);
+            continue;
+        }
+        rest.push_back(kv);
+    }
+    auto table = [&](uint32_t bits, const std::vector<std::pair<uint32_t, cd>> &ops) {
+        if (g.coef->size() & 1) g.push(0.0);
+        const uint32_t i0 = (uint32_t)g.coef->size();
+        for (uint32_t v = 0; v < (1u << bits); v++) {
+            cd t(1.0, 0.0);
+            for (auto &op : ops)
+                if ((v & op.first) == op.first) t *= op.second;
+            g.push(t.real());
+            g.push(t.imag());
+        }
+        return i0;
+    };
+    if (thr.size() >= 2) {
+        const uint32_t i0 = table(g.LT, thr);
+        o << "  V " << d << "; " << d << ".x = c[" << i0 << "u + 2u*tid]; " << d << ".y = c[" << i0 << "u + 2u*tid + 1u];\n";
+    } else {
+        rest.insert(rest.end(), thr_src.begin(), thr_src.end());
+        o << "  V " << d << "; " << d << ".x = 1; " << d << ".y = 0;\n";
+    }
+    for (auto &kv : grp) {
+        if (kv.second.size() < 2) {
+            rest.insert(rest.end(), grp_src[kv.first].begin(), grp_src[kv.first].end());
+            continue;
+        }
+        const uint32_t gi = kv.first;
+        const uint32_t bits = std::min<uint32_t>(8, (uint32_t)g.oq.size() - 8 * gi);
+        const uint32_t i0 = table(bits, kv.second);
+        o << "  { const unsigned ig = (unsigned)(b >> " << 8 * gi << ") & " << ((1u << bits) - 1) << "u; const T tr = c["
+          << i0 << "u + 2u*ig], ti = c[" << i0 << "u + 2u*ig + 1u];\n";
+        emit_cmul_var(o, d, "tr", "ti", "    ");
+        o << "  }\n";
+    }
+    for (auto &kv : rest) {
+        const cd c = kv.second;
+        o << "  if " << cond_expr(g, kv.first, cur) << " {\n";
+        if (c == cd(0.0, 1.0) || c == cd(0.0, -1.0)) {
+            o << "    { const V x = " << d << "; ";
+            ipow_assign(o, d, "x", c.imag() > 0 ? 1 : 3);
+            o << " }\n";
+        } else {
+            const uint32_t kc = g.push(c.real());
+            g.push(c.imag());
+            emit_cmul_var(o, d, g.C(kc), g.C(kc + 1), "    ");
+        }
+        o << "  }\n";
+    }
+    return d;
+}
+
+// XOR of a sign-bit mask into both components of v (exact negation where set).
+void emit_sign_xor(Gen &g, const std::string &v, const std::string &mask, const char *pre) {
+    auto &o = g.o;
+    if (g.dbl) {
+        for (const char *c : {".x", ".y"})
+            o << pre << v << c << " = __hiloint2double(__double2hiint(" << v << c << ") ^ (int)" << mask << ", __double2loint("
+              << v << c << "));\n";
+    } else {
+        for (const char *c : {".x", ".y"})
+            o << pre << v << c << " = __int_as_float(__float_as_int(" << v << c << ") ^ (int)" << mask << ");\n";
+    }
+}
+
+
+// Multiplies every register holding mask m by the pending factor p.
+void apply_pending(Gen &g, uint32_t m, const Pending &p, uint32_t cur) {
+    auto &o = g.o;
+    auto exact = [](cd c, double re, double im) { return c.real() == re && c.imag() == im; };
+    if (p.rt < 0 && p.sg >= 0 && (exact(p.k, 1.0, 0.0) || exact(p.k, -1.0, 0.0))) {
+        // a sign per thread (and a constant sign): one XOR per component
+        std::string mask = "S" + std::to_string(p.sg);
+        if (exact(p.k, -1.0, 0.0)) {
+            o << "  " << mask << " ^= 0x80000000u;\n";
+        }
+        for (uint32_t r = 0; r < g.NR; r++)
+            if ((r & m) == m) emit_sign_xor(g, g.var(r), mask, "  ");
+        return;
+    }
+    if (p.rt < 0 && p.sg >= 0) {
+        // a general constant with a per-thread sign: the signed constant per thread
+        const uint32_t k = g.push(p.k.real());
+        g.push(p.k.imag());
+        o << "  V DS" << p.sg << "; DS" << p.sg << ".x = " << g.C(k) << "; DS" << p.sg << ".y = " << g.C(k + 1) << ";\n";
+        emit_sign_xor(g, "DS" + std::to_string(p.sg), "S" + std::to_string(p.sg), "  ");
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if ((r & m) != m) continue;
+            const std::string a = g.var(r), d = "DS" + std::to_string(p.sg);
+            o << "  { const V x = " << a << "; " << a << ".x = " << d << ".x*x.x - " << d << ".y*x.y; " << a << ".y = "
+              << d << ".x*x.y + " << d << ".y*x.x; }\n";
+        }
+        return;
+    }
+    if (p.rt < 0) {
+        const cd c = p.k;
+        if (exact(c, 1.0, 0.0)) return;
+        const bool neg = exact(c, -1.0, 0.0), pi = exact(c, 0.0, 1.0), mi = exact(c, 0.0, -1.0);
+        uint32_t kr = 0;
+        if (!neg && !pi && !mi) {
+            kr = g.push(c.real());
+            g.push(c.imag());
+        }
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if ((r & m) != m) continue;
+            const std::string a = g.var(r);
+            if (neg)
+                o << "  " << a << ".x = -" << a << ".x; " << a << ".y = -" << a << ".y;\n";
+            else if (pi || mi) {
+                o << "  { const V x = " << a << "; ";
+                ipow_assign(o, a, "x", pi ? 1 : 3);
+                o << " }\n";
+            } else
+                o << "  { const V x = " << a << "; " << a << ".x = " << g.C(kr) << "*x.x - " << g.C(kr + 1) << "*x.y; "
+                  << a << ".y = " << g.C(kr) << "*x.y + " << g.C(kr + 1) << "*x.x; }\n";
+        }
+        return;
+    }
+    const std::string d = build_runtime(g, p.conds, cur, p.rt);
+    if (p.sg >= 0) emit_sign_xor(g, d, "S" + std::to_string(p.sg), "  ");
+    if (!exact(p.k, 1.0, 0.0)) {
+        const uint32_t k = g.push(p.k.real());
+        g.push(p.k.imag());
+        emit_cmul_var(o, d, g.C(k), g.C(k + 1), "  ");
+    }
+    for (uint32_t r = 0; r < g.NR; r++) {
+        if ((r & m) != m) continue;
+        const std::string a = g.var(r);
+        o << "  { const V x = " << a << "; " << a << ".x = " << d << ".x*x.x - " << d << ".y*x.y; " << a << ".y = "
+          << d << ".x*x.y + " << d << ".y*x.x; }\n";
+    }
+}
+
+uint32_t swap_bits(uint32_t m, uint32_t a, uint32_t b) {
+    const uint32_t ba = (m >> a) & 1u, bb = (m >> b) & 1u;
+    if (ba == bb) return m;
+    return m ^ ((1u << a) | (1u << b));
+}
+
+void emit_stages_lazy(Gen &g, const SweepH &sw, uint32_t &cur, uint32_t memr) {
+    auto &o = g.o;
+    std::map<uint32_t, Pending> pend;
+    int nvars = 0;
+    auto flush = [&](uint32_t touched, bool all) {
+        for (auto it = pend.begin(); it != pend.end();) {
+            if (!all && !(it->first & touched)) {
+                ++it;
+                continue;
+            }
+            apply_pending(g, it->first, it->second, cur);
+            it = pend.erase(it);
+        }
+    };
+    for (auto &St : sw.stages) {
+        if (St.rmask_pos != cur) {
+            flush(0, true);
+            emit_exchange(g, cur, St.rmask_pos);
+            cur = St.rmask_pos;
+        }
+        for (uint32_t k = St.op_begin; k < St.op_end; k++) {
+            const TileOpH &op = sw.ops[k];
+            switch (op.kind) {
+            case TK_DM: {
+                Pending &p = pend[op.rmask];
+                if (!op.gmask) {
+                    p.k *= op.m[0];
+                    break;
+                }
+                const cd c = op.m[0];
+                if (c == cd(-1.0, 0.0)) {
+                    // a conditioned sign flip: a sign-bit mask per thread
+                    if (p.sg < 0) {
+                        p.sg = nvars++;
+                        o << "  unsigned S" << p.sg << " = 0u;\n";
+                    }
+                    o << "  if " << cond_expr(g, op.gmask, cur) << " S" << p.sg << " ^= 0x80000000u;\n";
+                    break;
+                }
+                // a conditioned phase: multiplied into the factor when it is applied
+                if (p.rt < 0) p.rt = nvars++;
+                p.conds.push_back({op.gmask, c});
+                break;
+            }
+            case TK_SWAPR: {
+                // the pending masks follow the renamed register bits
+                std::map<uint32_t, Pending> moved;
+                for (auto &kv : pend) moved[swap_bits(kv.first, op.j0, op.j1)] = kv.second;
+                pend.swap(moved);
+                emit_op(g, op, cur);
+                break;
+            }
+            case TK_U2: flush((1u << op.j0) | (1u << op.j1), false); emit_op(g, op, cur); break;
+            default: flush(1u << op.j0, false); emit_op(g, op, cur); break;
+            }
+        }
+    }
+    flush(0, true);
+    if (cur != memr) {
+        emit_exchange(g, cur, memr);
+        cur = memr;
+    }
+}
+
 }  // namespace
 
 std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef,
@@ -474,14 +939,18 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     }
     auto &o = g.o;
     const int S = dbl ? 4 : 5;
+    g.S = S;
+    g.half = use_half();
+    const int waves = jit_waves();
     o << "typedef " << (dbl ? "double" : "float") << " T;\ntypedef " << (dbl ? "double2" : "float2") << " V;\n";
     o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
       << ")) & " << ((1u << S) - 1) << "u); }\n";
-    o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
-      << ") void " << (from_basis ? "ncsv_sweep_from_basis" : "ncsv_sweep")
+    o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT);
+    if (waves) o << ", " << waves;
+    o << ") void " << (from_basis ? "ncsv_sweep_from_basis" : "ncsv_sweep")
       << "(V *__restrict__ amp, const T *__restrict__ c" << (from_basis ? ", unsigned long long binit" : "")
       << ") {\n";
-    if (lds) o << "  __shared__ V lds[" << (1u << g.L) << "];\n";
+    if (lds) o << "  __shared__ V lds[" << (1u << (g.half ? g.L - 1 : g.L)) << "];\n";
     o << "  const unsigned tid = threadIdx.x;\n  const unsigned long long b = blockIdx.x;\n";
     o << "  const unsigned long long base = " << deposit("b", g.oq, true) << ";\n";
     std::vector<uint32_t> low(sw.tq.begin(), sw.tq.begin() + g.LT);
@@ -506,6 +975,9 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
             o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
     }
     uint32_t cur = memr;
+    if (use_lazy_diagonals()) {
+        emit_stages_lazy(g, sw, cur, memr);
+    } else {
     for (auto &St : sw.stages) {
         if (St.rmask_pos != cur) {
             emit_exchange(g, cur, St.rmask_pos);
@@ -531,14 +1003,39 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
         flush();
     }
     if (cur != memr) emit_exchange(g, cur, memr);
+    }
     if (g.scale != 1.0) {
         // the factors taken out of the real-form 2x2s
         const std::string S = g.C(g.push(g.scale));
         for (uint32_t r = 0; r < g.NR; r++)
             o << "  " << g.var(r) << ".x *= " << S << "; " << g.var(r) << ".y *= " << S << ";\n";
     }
-    for (uint32_t r = 0; r < g.NR; r++) o << "  amp[gm | " << roff(r) << "ull] = " << g.var(r) << ";\n";
+    {
+        // store addresses recomputed from an opaque copy of the thread index,
+        // so the load addresses are not held in registers across the sweep
+        std::ostringstream e;
+        e << "base";
+        for (uint32_t i = 0; i < g.LT; i++)
+            e << " | (((unsigned long long)(tids >> " << i << ") & 1ull) << " << low[i] << ")";
+        o << "  unsigned tids = tid;\n  asm volatile(\"\" : \"+v\"(tids));\n";
+        o << "  const unsigned long long gs = " << e.str() << ";\n";
+    }
+    for (uint32_t r = 0; r < g.NR; r++) o << "  amp[gs | " << roff(r) << "ull] = " << g.var(r) << ";\n";
     o << "}\n";
+    if (const char *dir = getenv("NCSV_JIT_DUMP")) {
+        // the sweep's coefficients beside its dumped source (doubles, in order)
+        const std::string path =
+            std::string(dir) + "/ncsv_" + std::to_string(std::hash<std::string>()(o.str())) + ".coef";
+        if (FILE *f = fopen(path.c_str(), "wb")) {
+            fwrite(coef->data(), sizeof(double), coef->size(), f);
+            fclose(f);
+        }
+        // and the order the sweeps were generated in
+        if (FILE *f = fopen((std::string(dir) + "/order.txt").c_str(), "a")) {
+            fprintf(f, "%zu %d\n", std::hash<std::string>()(o.str()), from_basis ? 1 : 0);
+            fclose(f);
+        }
+    }
     return o.str();
 }
 
@@ -554,6 +1051,8 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
     Gen g;
     // each exchange's LDS addresses are computed where it runs, not held across the tile loop
     g.opaque = true;
+    // NCSV_EXPECT_HALF=1: exchanges through half a tile of LDS, as NCSV_JIT_HALF for sweeps
+    if (const char *e = getenv("NCSV_EXPECT_HALF")) g.half = atoi(e);
     std::vector<double> dummy;
     g.coef = &dummy;
     g.LT = geo.LT;
@@ -581,13 +1080,14 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
     }
     auto &o = g.o;
     const int S = dbl ? 4 : 5;
+    g.S = S;
     o << "typedef " << (dbl ? "double" : "float") << " T;\ntypedef " << (dbl ? "double2" : "float2") << " V;\n";
     o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
       << ")) & " << ((1u << S) - 1) << "u); }\n";
     o << "__device__ __forceinline__ double wsum(double v) { for (int k = 32; k > 0; k >>= 1) v += __shfl_down(v, k, 64); return v; }\n";
     o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
       << ") void ncsv_expect(const V *__restrict__ amp, double *__restrict__ part, unsigned long long ntiles) {\n";
-    if (lds) o << "  __shared__ V lds[" << (1u << g.L) << "];\n";
+    if (lds) o << "  __shared__ V lds[" << (1u << (g.half ? g.L - 1 : g.L)) << "];\n";
     o << "  __shared__ double red[" << ((1u << g.LT) / 64) << "][" << 2 * cnt << "];\n";
     o << "  const unsigned tid = threadIdx.x;\n";
     for (uint32_t k = 0; k < cnt; k++) o << "  double sr" << k << " = 0.0, si" << k << " = 0.0;\n";

@@ -39,6 +39,8 @@ def lib():
         L.orc_sample.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int64, ctypes.POINTER(u64)]
         L.orc_sample.restype = ctypes.c_int
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        L.orc_set_num_threads.restype = None
         _lib = L
     return _lib
 

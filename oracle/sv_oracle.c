@@ -533,6 +533,15 @@ int orc_sample(const double *amp_, int n, const double *uniforms, int64_t shots,
     return 0;
 }
 
+/* Thread count of the OpenMP loops (the CPU baseline times 1 thread and all). */
+void orc_set_num_threads(int t) {
+#ifdef _OPENMP
+    omp_set_num_threads(t > 0 ? t : 1);
+#else
+    (void)t;
+#endif
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

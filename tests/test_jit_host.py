@@ -1,0 +1,149 @@
+"""CPU check of the specialized sweep kernels: the HIP source the generator
+writes for every sweep of a plan is compiled for the host with g++ (HIP's
+thread and block indices, __syncthreads and __shared__ mapped onto host
+threads and a barrier), run one workgroup at a time on 256 host threads,
+and compared with the plan's FP64 host emulation (ncsv_plan_emulate).  This
+exercises the generator's register renaming, every LDS exchange form
+(half-tile passes on a shared register position, wavefront turns,
+intermediate layouts, whole-tile), the lazily applied diagonal factors and
+their thread and block-index tables, without a GPU."""
+import ctypes
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import ncsim_amd as H
+from ncsim_amd import _lib as L
+from ncsim_amd import workloads as W
+
+PRE = r'''
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+struct double2 { double x, y; };
+struct float2 { float x, y; };
+struct TI { unsigned x; };
+struct BI { unsigned long long x; };
+thread_local TI threadIdx;
+thread_local BI blockIdx;
+static std::barrier<> *g_bar;
+#define __syncthreads() g_bar->arrive_and_wait()
+#define __global__
+#define __device__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __restrict__
+#define __shared__ static
+#define __popcll __builtin_popcountll
+inline int __double2hiint(double d) { long long b; memcpy(&b, &d, 8); return (int)(b >> 32); }
+inline int __double2loint(double d) { long long b; memcpy(&b, &d, 8); return (int)(unsigned)(b & 0xffffffffu); }
+inline double __hiloint2double(int hi, int lo) {
+    long long b = ((long long)hi << 32) | (unsigned)lo; double d; memcpy(&d, &b, 8); return d; }
+inline int __float_as_int(float f) { int i; memcpy(&i, &f, 4); return i; }
+inline float __int_as_float(int i) { float f; memcpy(&f, &i, 4); return f; }
+'''
+
+POST = r'''
+extern "C" void run_host(V *amp, unsigned long long nblocks, unsigned nthreads, const T *coef) {
+    for (unsigned long long b = 0; b < nblocks; b++) {
+        std::barrier<> bar(nthreads);
+        g_bar = &bar;
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nthreads; t++)
+            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_sweep(amp, coef); });
+        for (auto &x : th) x.join();
+    }
+}
+'''
+
+
+def host_kernel(src, workdir, name):
+    src = "\n".join(line for line in src.splitlines() if "asm volatile" not in line)
+    src = src.replace('extern "C" ', "")
+    cpp = os.path.join(workdir, name + ".cpp")
+    with open(cpp, "w") as f:
+        f.write(PRE + src + POST)
+    so = os.path.join(workdir, name + ".so")
+    r = subprocess.run(["g++", "-std=c++20", "-O1", "-shared", "-fPIC", "-o", so, cpp, "-lpthread"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[:4000]
+    return ctypes.CDLL(so)
+
+
+def run_plan_on_host(text, precision, monkeypatch, half):
+    """Every sweep kernel of the circuit's plan, in order, on the host;
+    returns (host result, emulated result, plan stats)."""
+    work = tempfile.mkdtemp()
+    try:
+        monkeypatch.setenv("NCSV_JIT_DUMP", work)
+        monkeypatch.setenv("NCSV_JIT_HALF", str(half))
+        c = H.Circuit.parse(text)
+        n = c.num_qubits()
+        k = ctypes.c_uint32()
+        assert L.lib().ncsv_jit_check(c._h, int(precision), b"gfx950", ctypes.byref(k)) == 0
+        order = [line.split() for line in open(os.path.join(work, "order.txt")).read().splitlines()]
+        hashes = [h for h, from_basis in order if from_basis == "0"]
+        assert len(hashes) == k.value
+        rng = np.random.default_rng(n)
+        a = (rng.random(1 << n) - 0.5) + 1j * (rng.random(1 << n) - 0.5)
+        want = a.copy()
+        stats = (ctypes.c_uint64 * 4)()
+        assert L.lib().ncsv_plan_emulate(c._h, int(precision), want.view(np.float64).ctypes.data_as(L.c_dp),
+                                         stats) == 0
+        buf = a.copy() if precision == H.C128 else a.astype(np.complex64)
+        tile = 12 if precision == H.C128 else 13
+        for i, h in enumerate(hashes):
+            src = open(os.path.join(work, f"ncsv_{h}.hip")).read()
+            coef = np.fromfile(os.path.join(work, f"ncsv_{h}.coef"), dtype=np.float64)
+            coef = np.ascontiguousarray(coef if precision == H.C128 else coef.astype(np.float32))
+            lib = host_kernel(src, work, f"k{i}")
+            lib.run_host(buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_ulonglong(1 << (n - tile)), ctypes.c_uint(256),
+                         coef.ctypes.data_as(ctypes.c_void_p))
+        return buf.astype(np.complex128), want, list(stats)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def outer_heavy(n, tile, seed):
+    """Gates on the low `tile` qubits plus phases and CZs reaching the
+    others: one sweep on a multi-tile state whose diagonal factors depend on
+    block-index bits."""
+    r = np.random.default_rng(seed)
+    lines = [f"QUBITS {n}"]
+    for _ in range(4):
+        for q in range(tile):
+            lines.append(f"{['H', 'RX(0.3)', 'RY(1.1)', 'SQRT_X', 'T'][r.integers(5)]} {q}")
+        for q in range(0, tile - 1, 2):
+            lines.append(f"{'CX' if r.random() < 0.5 else 'CZ'} {q} {q + 1}")
+        for q in range(tile, n):
+            lines.append(f"RZ({r.random() * 6!r}) {q}")
+            lines.append(f"CZ {q} {int(r.integers(tile))}")
+            lines.append(f"CPHASE(1,{int(r.integers(2, 30))}) {q} {int(r.integers(tile))}")
+    return "\n".join(lines) + "\n"
+
+
+CASES = [
+    ("brickwork", H.C128, lambda: W.rotation_brickwork(12, 3, 0), 1),
+    ("qft", H.C128, lambda: W.qft_with_rz(14, 12, 1), 1),
+    ("qft-c64", H.C64, lambda: W.qft_with_rz(15, 12, 2), 1),
+    ("brickwork-c64", H.C64, lambda: W.rotation_brickwork(15, 4, 3), 1),
+    ("outer", H.C128, lambda: outer_heavy(21, 12, 4), 1),
+    ("outer-c64", H.C64, lambda: outer_heavy(15, 13, 5), 1),
+    ("wavefront-turns", H.C128, lambda: W.rotation_brickwork(14, 4, 6), 2),
+    ("whole-tile", H.C64, lambda: W.qft_with_rz(13, 8, 7), 0),
+]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ with C++20")
+@pytest.mark.parametrize("name,precision,make,half", CASES, ids=[c[0] for c in CASES])
+def test_generated_sweeps_on_host_threads(name, precision, make, half, monkeypatch):
+    got, want, stats = run_plan_on_host(make(), precision, monkeypatch, half)
+    tol = 1e-13 if precision == H.C128 else 2e-6
+    assert np.abs(got - want).max() < tol, (name, stats)
