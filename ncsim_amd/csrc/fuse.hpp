@@ -78,6 +78,11 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
 std::vector<SweepH> plan_lds_sweeps(const std::vector<LOp> &ops, uint32_t n, uint32_t L, uint32_t c,
                                     uint64_t first_fixed);
 
+// plan_segment; with `specialized`, also with one and two more fixed low
+// qubits (longer contiguous runs), keeping the plan of least estimated cost
+// and setting *geo to its shape.
+std::vector<SegItem> plan_segment_tuned(const std::vector<LOp> &ops, uint32_t n, TileGeom *geo, bool specialized);
+
 // Whether the state runs kernels specialized per plan (fusion 2, or fusion 1
 // from NCSV_JIT_MIN_QUBITS = 20 qubits on; never with NCSV_JIT=0).
 bool use_specialized(const ncsv_state *s);

@@ -1519,7 +1519,7 @@ ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t coun
 ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const char *arch, uint32_t *nsweeps) {
     try {
         using namespace ncsv;
-        const TileGeom geo = dev::tile_geom(prec);
+        TileGeom geo = dev::tile_geom(prec);
         std::vector<LOp> ops;
         for (const COp &op : c->ops) {
             if (op.kind == CKind::Gate)
@@ -1527,7 +1527,7 @@ ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const cha
             else if (op.kind != CKind::Tick)
                 fail(NCSV_INVALID, "jit check takes unitary circuits");
         }
-        const std::vector<SegItem> items = plan_segment(ops, c->n, geo);
+        const std::vector<SegItem> items = plan_segment_tuned(ops, c->n, &geo, true);
         uint32_t k = 0;
         for (const SegItem &it : items) {
             if (!it.tile) continue;
