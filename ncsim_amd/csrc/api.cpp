@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -54,9 +55,21 @@ void check_register(const ncsv_state *s, uint64_t x, uint64_t z, const char *wha
     if ((x | z) & ~reg_mask(s->n)) fail(NCSV_INVALID, std::string(what) + ": operator on the wrong register");
 }
 
+// NCSV_DEVICE_MEMORY_LIMIT_GB caps the bytes one state may take (testing the
+// out-of-memory path without filling the device); unset, no cap.
+void check_memory_cap(size_t bytes) {
+    const char *e = getenv("NCSV_DEVICE_MEMORY_LIMIT_GB");
+    if (!e) return;
+    const double cap = atof(e) * (double)(1ull << 30);
+    if ((double)bytes > cap)
+        fail(NCSV_DEVICE, "out of device memory: " + std::to_string(bytes) + " bytes requested, NCSV_DEVICE_MEMORY_LIMIT_GB=" +
+                              std::string(e));
+}
+
 ncsv_state *alloc_state(uint32_t n, ncsv_precision prec) {
     if (n > 34) fail(NCSV_INVALID, "state vector of more than 34 qubits");
     if (prec != NCSV_C128 && prec != NCSV_C64) fail(NCSV_INVALID, "unknown precision");
+    check_memory_cap((size_t(1) << n) * (prec == NCSV_C128 ? 16 : 8));
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(NCSV_DEVICE, "no GPU device visible");
     auto s = std::make_unique<ncsv_state>();

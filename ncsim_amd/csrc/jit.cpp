@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <future>
+#include <list>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -1200,10 +1202,31 @@ struct Entry {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     std::string err;
+    // the module goes when the last plan or cache slot holding it does
+    ~Entry() {
+        if (mod) (void)hipModuleUnload(mod);
+    }
 };
 
+// Compiled kernels by source text, least recently used first out: at most
+// NCSV_JIT_CACHE (default 1024) finished entries stay cached.  Plans hold
+// their own references, so an evicted kernel in use stays loaded until its
+// plan goes; expectation kernels (whose sources hold the Paulis' masks) of
+// observables no longer asked for are unloaded instead of accumulating.
 std::mutex g_mu;
-std::map<std::string, std::shared_future<std::shared_ptr<Entry>>> g_cache;
+struct Slot {
+    std::shared_future<std::shared_ptr<Entry>> f;
+    std::list<std::string>::iterator lru;
+};
+// held for the life of the process (no module unloads during static
+// destruction, after the runtime may have gone)
+std::map<std::string, Slot> &g_cache = *new std::map<std::string, Slot>();
+std::list<std::string> &g_lru = *new std::list<std::string>();  // front: most recently used
+
+size_t cache_cap() {
+    const char *e = getenv("NCSV_JIT_CACHE");
+    return e ? (size_t)std::max(1, atoi(e)) : 1024;
+}
 
 std::string arch_name() {
     int dev = 0;
@@ -1256,7 +1279,18 @@ std::vector<char> compile_to_code(const std::string &src, const std::string &arc
 std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src, const char *fname = "ncsv_sweep") {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_cache.find(src);
-    if (it != g_cache.end()) return it->second;
+    if (it != g_cache.end()) {
+        g_lru.splice(g_lru.begin(), g_lru, it->second.lru);
+        return it->second.f;
+    }
+    // evict the least recently used finished compiles beyond the cap
+    for (auto r = g_lru.end(); g_cache.size() >= cache_cap() && r != g_lru.begin();) {
+        --r;
+        auto c = g_cache.find(*r);
+        if (c->second.f.wait_for(std::chrono::seconds(0)) != std::future_status::ready) continue;
+        g_cache.erase(c);
+        r = g_lru.erase(r);
+    }
     const std::string arch = arch_name();
     const std::string fn = fname;
     std::shared_future<std::shared_ptr<Entry>> f = std::async(std::launch::async, [src, arch, fn]() {
@@ -1275,7 +1309,8 @@ std::shared_future<std::shared_ptr<Entry>> get_or_start(const std::string &src, 
                                                          }
                                                          return e;
                                                      }).share();
-    g_cache[src] = f;
+    g_lru.push_front(src);
+    g_cache[src] = Slot{f, g_lru.begin()};
     return f;
 }
 

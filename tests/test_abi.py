@@ -91,3 +91,19 @@ def test_circuit_builder_checks():
     c.measure_z(2).reset(1).noise("DEPOLARIZE1", [0.01], [0]).tick()
     assert c.num_gates() == 3 and c.num_measurements() == 1
     assert "RZ(0.5) 2" in str(c)
+
+
+def test_out_of_device_memory_is_a_device_error(monkeypatch):
+    """A state larger than NCSV_DEVICE_MEMORY_LIMIT_GB fails with the Device
+    status and no handle (the path a full device takes), with or without a
+    GPU; clone and tensor allocate through the same check."""
+    monkeypatch.setenv("NCSV_DEVICE_MEMORY_LIMIT_GB", "0.001")
+    h = ctypes.c_void_p()
+    st = L.lib().ncsv_create(17, 0, ctypes.byref(h))  # 2 MiB of complex128
+    assert st == 9 and not h.value
+    buf = ctypes.create_string_buffer(256)
+    L.lib().ncsv_last_error(buf, 256)
+    assert b"out of device memory" in buf.value
+    with pytest.raises(ncsim_amd.NcsvError) as e:
+        ncsim_amd.HipStateVector.zero_state(20, ncsim_amd.C64)
+    assert e.value.kind == "Device"

@@ -621,3 +621,44 @@ def test_measuring_a_zero_state_follows_the_reference(precision):
     assert o.measure(Pauli.parse("Z1", 3), ro) is False
     assert d.measure(Pauli.parse("Z1", 3), rd) is False
     assert ro.gen_f64() == rd.gen_f64()  # the same number of draws on both sides
+
+
+def test_jit_cache_evicts_without_breaking_plans(monkeypatch):
+    """With room for three compiled kernels, new expectation kernels evict
+    old ones (unloading their modules) while a circuit's plan keeps its
+    kernels: expectations of cycling Pauli sets and a re-run of the planned
+    circuit stay equal to the oracle."""
+    monkeypatch.setenv("NCSV_JIT_CACHE", "3")
+    n = 20
+    text = W.rotation_brickwork(n, 6, 21)
+    o = StateVector.zero_state(n)
+    o.run(OCircuit.parse(text), SplitMix64(0))
+    circ = H.Circuit.parse(text)
+    d = H.HipStateVector.zero_state(n)
+    d.run(circ)
+    sets = [[Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 6, 30 + k)] for k in range(5)]
+    for rnd in range(2):
+        for ps in sets:
+            got = d.expectations(ps)
+            assert max(abs(o.expectation(p) - g) for p, g in zip(ps, got)) < TOL
+        d.set_basis(0)
+        d.run(circ)
+        assert maxerr(o, d) < TOL
+
+
+def test_out_of_device_memory_cap_on_the_device(monkeypatch):
+    """NCSV_DEVICE_MEMORY_LIMIT_GB: a state above the cap is a Device error
+    with no handle; states below it, and the handles made before, work."""
+    a = H.HipStateVector.zero_state(12)
+    monkeypatch.setenv("NCSV_DEVICE_MEMORY_LIMIT_GB", "0.25")
+    with pytest.raises(H.NcsvError) as e:
+        H.HipStateVector.zero_state(24)  # 256 MiB of complex128
+    assert e.value.kind == "Device"
+    with pytest.raises(H.NcsvError) as e:
+        H.HipStateVector.zero_state(12).tensor(H.HipStateVector.zero_state(12))
+    assert e.value.kind == "Device"
+    b = H.HipStateVector.zero_state(23, H.C64)  # 64 MiB
+    b.apply_gate("H", [22])
+    assert abs(b.norm_sq() - 1) < 1e-6
+    a.apply_gate("X", [0])
+    assert abs(a.amplitude(1) - 1) == 0
