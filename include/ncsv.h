@@ -320,6 +320,41 @@ ncsv_status ncsv_batch_emulate(ncsv_batch *b, const double *uniforms, const uint
  * simulated by host threads (no device; small batches only). */
 ncsv_status ncsv_batch_simulate(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, const uint8_t *forced);
 
+/* ------------------------------------------------------------ frame sampler
+ * Pauli-frame sampling of noisy Clifford circuits (src/frame.rs
+ * FrameSampler over the column layout of src/pauli/columns.rs): one
+ * noiseless reference record from a stabilizer tableau, then per shot that
+ * record XOR the anticommutation of an accumulated Pauli frame with each
+ * measured Pauli; 64 shots per 64-bit word.  Draws are counter-based: the
+ * caller's seed plays the part of the reference's `rng.next_u64()` per
+ * sample call, and results do not depend on how shots are split over
+ * threads.  Measured Paulis lie on qubits 0..63 (as the circuit format
+ * allows); gates and noise on any qubit. */
+typedef struct ncsv_frames ncsv_frames;
+
+/* FrameSampler::new (frame.rs:41-58): Unsupported for a non-Clifford gate;
+ * the reference record's random outcomes take the top bits of successive
+ * SplitMix64 outputs of reference_seed. */
+ncsv_status ncsv_frames_new(const ncsv_circuit *c, uint64_t reference_seed, ncsv_frames **out);
+ncsv_status ncsv_frames_destroy(ncsv_frames *f);
+/* Circuit::num_measurements, detectors().len(), observables().len()
+ * (circuit.rs:297-327) of the sampler's circuit. */
+ncsv_status ncsv_frames_shape(const ncsv_frames *f, uint64_t *measurements, uint64_t *detectors,
+                              uint64_t *observables);
+/* FrameSampler::reference (frame.rs:66-69): record[k] = outcome k. */
+ncsv_status ncsv_frames_reference(const ncsv_frames *f, uint8_t *record, uint64_t cap);
+/* FrameSampler::sample_packed (frame.rs:72-92): out[k * W + w] holds
+ * measurement k of shots 64w..64w+63 (bit s = shot 64w + s), W =
+ * ceil(shots / 64), bits past the last shot zero. */
+ncsv_status ncsv_frames_sample_packed(ncsv_frames *f, uint64_t shots, uint64_t seed, uint64_t *out);
+/* FrameSampler::sample_detectors (frame.rs:150-211): detector and
+ * observable flips, laid out as sample_packed's records. */
+ncsv_status ncsv_frames_sample_detectors(ncsv_frames *f, uint64_t shots, uint64_t seed, uint64_t *det,
+                                         uint64_t *obs);
+/* Diagnostics (no counterpart in the reference): sample_packed run on the
+ * host by the kernel's own per-word code (no device). */
+ncsv_status ncsv_frames_emulate_packed(const ncsv_frames *f, uint64_t shots, uint64_t seed, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -566,3 +566,61 @@ class HipBatch:
         keys = ("kernel_ms", "plan_ms", "items", "launches", "amp_updates", "hbm_bytes", "programs", "steps",
                 "graph_replays")
         return dict(zip(keys, list(out)))
+
+
+class HipFrameSampler:
+    """FrameSampler (src/frame.rs): Pauli-frame sampling of a noisy Clifford
+    circuit, 64 shots per word, on the GPU (include/ncsv.h, frame sampler).
+    `seed` arguments replace the reference's `rng` (one u64 per call)."""
+
+    def __init__(self, circuit, reference_seed=0):
+        h = ctypes.c_void_p()
+        _check(L.lib().ncsv_frames_new(circuit._h, int(reference_seed) & (2 ** 64 - 1), ctypes.byref(h)))
+        self._h = h
+        m, d, o = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(L.lib().ncsv_frames_shape(self._h, ctypes.byref(m), ctypes.byref(d), ctypes.byref(o)))
+        self.num_measurements, self.num_detectors, self.num_observables = m.value, d.value, o.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and L._lib is not None:
+            L._lib.ncsv_frames_destroy(self._h)
+            self._h = None
+
+    def reference(self):
+        rec = (ctypes.c_uint8 * max(1, self.num_measurements))()
+        _check(L.lib().ncsv_frames_reference(self._h, rec, self.num_measurements))
+        return [int(rec[k]) for k in range(self.num_measurements)]
+
+    def sample_packed(self, shots, seed):
+        """Records packed by measurement: a list of uint64 arrays of
+        ceil(shots / 64) words (bit s of word w = shot 64 w + s)."""
+        import numpy as np
+        W = (shots + 63) // 64
+        out = np.zeros(max(1, self.num_measurements * W), dtype=np.uint64)
+        _check(L.lib().ncsv_frames_sample_packed(self._h, shots, int(seed) & (2 ** 64 - 1), out.ctypes.data_as(L.c_u64p)))
+        return [out[k * W:(k + 1) * W] for k in range(self.num_measurements)]
+
+    def emulate_packed(self, shots, seed):
+        """sample_packed run on the host by the kernel's per-word code."""
+        import numpy as np
+        W = (shots + 63) // 64
+        out = np.zeros(max(1, self.num_measurements * W), dtype=np.uint64)
+        _check(L.lib().ncsv_frames_emulate_packed(self._h, shots, int(seed) & (2 ** 64 - 1),
+                                                  out.ctypes.data_as(L.c_u64p)))
+        return [out[k * W:(k + 1) * W] for k in range(self.num_measurements)]
+
+    def sample(self, shots, seed):
+        """One record per shot (lists of bools), frame.rs:94-107."""
+        packed = self.sample_packed(shots, seed)
+        return [[bool((int(packed[k][s // 64]) >> (s % 64)) & 1) for k in range(len(packed))] for s in range(shots)]
+
+    def sample_detectors(self, shots, seed):
+        """(detectors, observables), each a list of packed uint64 arrays."""
+        import numpy as np
+        W = (shots + 63) // 64
+        det = np.zeros(max(1, self.num_detectors * W), dtype=np.uint64)
+        obs = np.zeros(max(1, self.num_observables * W), dtype=np.uint64)
+        _check(L.lib().ncsv_frames_sample_detectors(self._h, shots, int(seed) & (2 ** 64 - 1),
+                                                    det.ctypes.data_as(L.c_u64p), obs.ctypes.data_as(L.c_u64p)))
+        return ([det[k * W:(k + 1) * W] for k in range(self.num_detectors)],
+                [obs[k * W:(k + 1) * W] for k in range(self.num_observables)])

@@ -86,6 +86,13 @@ _SIGS = {
     "ncsv_batch_stats": [ctypes.c_void_p, c_dp],
     "ncsv_batch_plan": [ctypes.c_void_p],
     "ncsv_device_arch": [ctypes.c_char_p, ctypes.c_size_t],
+    "ncsv_frames_new": [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)],
+    "ncsv_frames_destroy": [ctypes.c_void_p],
+    "ncsv_frames_shape": [ctypes.c_void_p, c_u64p, c_u64p, c_u64p],
+    "ncsv_frames_reference": [ctypes.c_void_p, c_u8p, ctypes.c_uint64],
+    "ncsv_frames_sample_packed": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, c_u64p],
+    "ncsv_frames_sample_detectors": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, c_u64p, c_u64p],
+    "ncsv_frames_emulate_packed": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, c_u64p],
 }
 
 _lib = None

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libncsv.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["lower.cpp", "circuit.cpp", "api.cpp", "fuse.cpp", "jit.cpp", "batch.cpp", "batch_sim.cpp", "kernels.hip",
-           "tile.hip", "batch.hip"]
+           "tile.hip", "batch.hip", "frames.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NCSV_ARCH", "gfx950")

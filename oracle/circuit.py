@@ -110,6 +110,29 @@ class Circuit:
     def reset(self, q):
         return self.push(("reset", q))
 
+    def detectors(self):
+        """Circuit::detectors (circuit.rs:297-308): absolute measurement indices."""
+        m, out = 0, []
+        for o in self.ops:
+            if o[0] == "measure":
+                m += 1
+            elif o[0] == "detector":
+                out.append([m - k for k in o[1]])
+        return out
+
+    def observables(self):
+        """Circuit::observables (circuit.rs:311-327): OBSERVABLE_INCLUDE(i) lists
+        collected per index i."""
+        m, out = 0, []
+        for o in self.ops:
+            if o[0] == "measure":
+                m += 1
+            elif o[0] == "observable":
+                while len(out) <= o[1]:
+                    out.append([])
+                out[o[1]].extend(m - k for k in o[2])
+        return out
+
     def without_noise(self):
         c = Circuit(self.n)
         c.ops = [o for o in self.ops if o[0] != "noise"]

@@ -29,7 +29,7 @@ HOST_SAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=unde
 COMMON = ["-O1", "-g", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-I", os.path.join(ROOT, "include"),
           "-Wno-unused-function", "-Wno-unused-variable", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CPP = ["lower.cpp", "circuit.cpp", "api.cpp", "fuse.cpp", "jit.cpp", "batch.cpp", "batch_sim.cpp"]
-HIP = ["kernels.hip", "tile.hip", "batch.hip"]
+HIP = ["kernels.hip", "tile.hip", "batch.hip", "frames.hip"]
 
 
 def run(cmd):
