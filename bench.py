@@ -405,6 +405,50 @@ def run_traj(args):
     print(json.dumps(out), flush=True)
 
 
+def run_copy(args):
+    """Counter calibration: device-to-device copies of an n-qubit state
+    (ncsv_clone), each reading and writing 2^n amplitudes once."""
+    import ncsim_amd as H
+    prec = H.C64 if args.precision == "c64" else H.C128
+    s = H.HipStateVector.plus_state(args.qubits, prec)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c = s.clone()
+        c.synchronize()
+        del c
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    b = (1 << args.qubits) * (16 if prec == H.C128 else 8)
+    print(json.dumps({"workload": f"copy of a {args.qubits}-qubit state", "bytes_read": b, "bytes_written": b,
+                      "copies": args.steps, "ms_per_copy": ms}), flush=True)
+
+
+def run_frames(args):
+    """The Pauli-frame sampler: a distance-d repetition code (d data qubits,
+    d - 1 ancillas measured and reset every round, X_ERROR(p) on the data
+    before each round, detectors comparing consecutive rounds), sampled for
+    detectors and observables; shots/s including the records' download."""
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+    d, rounds, p = args.distance, args.rounds, 1e-3
+    shots = args.shots
+    text = W.repetition_code(d, rounds, p)
+    f = H.HipFrameSampler(H.Circuit.parse(text), 1)
+    for i in range(args.warmup):
+        f.sample_detectors(shots, 1000 + i)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        f.sample_detectors(shots, i)
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    print(json.dumps({"metric": "shots/s", "value": shots / (ms / 1000.0), "unit": "shots/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "u64 (bit-packed frames)",
+                      "data": "synthetic (seeded draws)",
+                      "config": {"workload": f"repetition code d={d}, {rounds} rounds, X_ERROR({p})",
+                                 "qubits": 2 * d - 1, "measurements": f.num_measurements,
+                                 "detectors": f.num_detectors, "shots": shots}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -416,7 +460,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--only-n", action="store_true", help="qft workload: only the n-qubit complex64 runs")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "qft", "batch", "trajectories"])
+    ap.add_argument("--workload", default="brickwork",
+                    choices=["brickwork", "qft", "batch", "trajectories", "frames", "copy"])
+    ap.add_argument("--distance", type=int, default=25, help="frames workload: repetition-code distance")
+    ap.add_argument("--rounds", type=int, default=25, help="frames workload: rounds")
+    ap.add_argument("--shots", type=int, default=1 << 24, help="frames workload: shots per step")
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--stride", type=int, default=8)
     ap.add_argument("--trajectories", type=int, default=2000)
@@ -439,6 +487,12 @@ def main():
         return
     if args.workload == "trajectories":
         run_traj(args)
+        return
+    if args.workload == "frames":
+        run_frames(args)
+        return
+    if args.workload == "copy":
+        run_copy(args)
         return
 
     import ncsim_amd as H

@@ -166,6 +166,27 @@ def noisy_brickwork(n, depth, seed, p=1e-3, every=10, mqubits=None):
     return "\n".join(lines) + "\n"
 
 
+def repetition_code(d, rounds, p):
+    """The reference's repetition-code memory experiment (frame.rs:401-442):
+    data qubits 0..d-1, ancilla d + a measures Z_a Z_{a+1} with two CXs and
+    is reset; X_ERROR(p) on every data qubit before each round; a round's
+    detectors compare each ancilla with the previous round (the first round
+    with nothing); the final data measurement's last bit is observable 0."""
+    text = f"QUBITS {2 * d - 1}\n"
+    for r in range(rounds):
+        text += f"X_ERROR({p})" + "".join(f" {q}" for q in range(d)) + "\n"
+        for a in range(d - 1):
+            anc = d + a
+            text += f"CX {a} {anc}\nCX {a + 1} {anc}\nM {anc}\nR {anc}\n"
+        for a in range(d - 1):
+            if r == 0:
+                text += f"DETECTOR rec[-{d - 1 - a}]\n"
+            else:
+                text += f"DETECTOR rec[-{d - 1 - a}] rec[-{2 * (d - 1) - a}]\n"
+    text += "M" + "".join(f" {q}" for q in range(d)) + "\nOBSERVABLE_INCLUDE(0) rec[-1]\n"
+    return text
+
+
 def count_gates(text):
     """Unfused gate applications of a text circuit (targets grouped by arity)."""
     ar = {"CX": 2, "CY": 2, "CZ": 2, "SWAP": 2, "CS": 2, "CS_DAG": 2, "CPHASE": 2, "CNOT": 2, "CCZ": 3, "CCX": 3,

@@ -9,7 +9,8 @@ import pytest
 import ncsim_amd as H
 from oracle import frames as F
 from oracle.circuit import Circuit
-from tests.test_frames_oracle import random_clifford_circuit, repetition_code
+from ncsim_amd.workloads import repetition_code
+from tests.test_frames_oracle import random_clifford_circuit
 
 
 def noisy_clifford(n, depth, seed, wide=0):

@@ -21,6 +21,7 @@ from oracle import frames as F
 from oracle.circuit import Circuit
 from oracle.pauli import Pauli, SimError
 from oracle.statevector import SplitMix64, StateVector, lib
+from ncsim_amd.workloads import repetition_code
 
 
 def histogram(records, m):
@@ -115,22 +116,6 @@ def test_tableau_records_are_possible_outcomes(seed):
                 v.apply_gate(Gate("X"), [op[1]])
             if op[0] == "measure":
                 k += 1
-
-
-def repetition_code(d, rounds, p):
-    text = f"QUBITS {2 * d - 1}\n"
-    for r in range(rounds):
-        text += f"X_ERROR({p})" + "".join(f" {q}" for q in range(d)) + "\n"
-        for a in range(d - 1):
-            anc = d + a
-            text += f"CX {a} {anc}\nCX {a + 1} {anc}\nM {anc}\nR {anc}\n"
-        for a in range(d - 1):
-            if r == 0:
-                text += f"DETECTOR rec[-{d - 1 - a}]\n"
-            else:
-                text += f"DETECTOR rec[-{d - 1 - a}] rec[-{2 * (d - 1) - a}]\n"
-    text += "M" + "".join(f" {q}" for q in range(d)) + "\nOBSERVABLE_INCLUDE(0) rec[-1]\n"
-    return text
 
 
 def test_repetition_code_detectors_and_observable():

@@ -12,7 +12,7 @@ import ncsim_amd as H
 from oracle import frames as F
 from oracle.circuit import Circuit
 from tests.test_frames_host import CASES
-from tests.test_frames_oracle import repetition_code
+from ncsim_amd.workloads import repetition_code
 
 pytestmark = pytest.mark.gpu
 
