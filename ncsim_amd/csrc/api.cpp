@@ -57,6 +57,8 @@ void check_register(const ncsv_state *s, uint64_t x, uint64_t z, const char *wha
 
 // NCSV_DEVICE_MEMORY_LIMIT_GB caps the bytes one state may take (testing the
 // out-of-memory path without filling the device); unset, no cap.
+// NCSV_DEVICE_MEMORY_LIMIT_GB caps one state's allocation, in units of 2^30
+// bytes (a state of exactly the cap is allowed).
 void check_memory_cap(size_t bytes) {
     const char *e = getenv("NCSV_DEVICE_MEMORY_LIMIT_GB");
     if (!e) return;

@@ -283,14 +283,14 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     const uint32_t h0 = lo | hibits(k0);
                     V v[CH];
 #pragma unroll
-                    for (uint32_t c = 0; c < CH; c++) v[c] = gt[h0 | hibits(c)];
+                    for (uint32_t c = 0; c < CH; c++) v[c] = NCSV_LDG(&gt[h0 | hibits(c)]);
 #pragma unroll
                     for (uint32_t c = 0; c < CH; c++) tile[st_tid ^ sw<V>(BT * (k0 + c))] = v[c];
                 }
             } else if (full) {
-                for (uint32_t k = 0; k < kcnt; k++) tile[st_tid ^ sw<V>(BT * k)] = gt[lo | hibits(k)];
+                for (uint32_t k = 0; k < kcnt; k++) tile[st_tid ^ sw<V>(BT * k)] = NCSV_LDG(&gt[lo | hibits(k)]);
             } else if (tid < NL) {
-                tile[st_tid] = gt[lo];
+                tile[st_tid] = NCSV_LDG(&gt[lo]);
             }
             NCSV_SYNC();
 
@@ -721,12 +721,12 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                         for (uint32_t k0 = 0; k0 < kcnt; k0 += CH) {
                             const uint32_t h0 = los | hibits(k0);
 #pragma unroll
-                            for (uint32_t c = 0; c < CH; c++) gs[h0 | hibits(c)] = val(st_tid ^ sw<V>(BT * (k0 + c)));
+                            for (uint32_t c = 0; c < CH; c++) NCSV_STG(&gs[h0 | hibits(c)], val(st_tid ^ sw<V>(BT * (k0 + c))));
                         }
                     } else if (full) {
-                        for (uint32_t k = 0; k < kcnt; k++) gs[los | hibits(k)] = val(st_tid ^ sw<V>(BT * k));
+                        for (uint32_t k = 0; k < kcnt; k++) NCSV_STG(&gs[los | hibits(k)], val(st_tid ^ sw<V>(BT * k)));
                     } else if (tid < NL) {
-                        gs[los] = val(st_tid);
+                        NCSV_STG(&gs[los], val(st_tid));
                     }
                 };
                 if (tpr != 1.0 || tpi != 0.0)

@@ -104,6 +104,8 @@ double shfl_xor(double x, unsigned mask) {
 #define NCSV_OPAQUE(x) ((void)0)
 #define NCSV_MFMA16(a, b, acc) ::ncsv::batch::sim::mfma16<float>((a), (b), (acc))
 #define NCSV_MFMA16D(a, b, acc) ::ncsv::batch::sim::mfma16<double>((a), (b), (acc))
+#define NCSV_LDG(p) (*(p))
+#define NCSV_STG(p, v) (*(p) = (v))
 #define NCSV_LANE0_STORE(dst, v) \
     do {                         \
         if (NCSV_TID == 0) (dst) = (v); \

@@ -650,7 +650,7 @@ def test_out_of_device_memory_cap_on_the_device(monkeypatch):
     """NCSV_DEVICE_MEMORY_LIMIT_GB: a state above the cap is a Device error
     with no handle; states below it, and the handles made before, work."""
     a = H.HipStateVector.zero_state(12)
-    monkeypatch.setenv("NCSV_DEVICE_MEMORY_LIMIT_GB", "0.25")
+    monkeypatch.setenv("NCSV_DEVICE_MEMORY_LIMIT_GB", "0.2")  # units of 2^30 bytes: 204.8 MiB
     with pytest.raises(H.NcsvError) as e:
         H.HipStateVector.zero_state(24)  # 256 MiB of complex128
     assert e.value.kind == "Device"

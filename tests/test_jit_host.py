@@ -64,8 +64,18 @@ extern "C" void run_host(V *amp, unsigned long long nblocks, unsigned nthreads, 
 '''
 
 
+HOST_NT = "inline V ld_nt(const V *p) { return *p; }\ninline void st_nt(V *p, V v) { *p = v; }"
+
+
 def host_kernel(src, workdir, name):
-    src = "\n".join(line for line in src.splitlines() if "asm volatile" not in line)
+    lines = []
+    for line in src.splitlines():
+        if "asm volatile" in line or "ext_vector_type" in line or "__builtin_nontemporal" in line:
+            continue  # the nontemporal helpers become plain accesses below
+        lines.append(line)
+        if line.startswith("typedef") and line.endswith(" V;") and "ld_nt(" in src:
+            lines.append(HOST_NT)
+    src = "\n".join(lines)
     src = src.replace('extern "C" ', "")
     cpp = os.path.join(workdir, name + ".cpp")
     with open(cpp, "w") as f:
