@@ -140,6 +140,40 @@ int main() {
         ncsv_batch_destroy(b);
         for (ncsv_circuit *cc : keep) ncsv_circuit_destroy(cc);
     }
+    // frame sampler: Clifford check, tableau reference record, op compiler
+    // and the kernel's per-word code on the host (a register over 64 qubits,
+    // and Pauli-product measurements on a small one)
+    {
+        const char *ts[2] = {
+            "QUBITS 70\nH 0\nCX 0 1\nS 1\nSQRT_Y 2\nCZ 2 0\nDEPOLARIZE1(0.1) 0 2\nDEPOLARIZE2(0.2) 1 2\n"
+            "M 0 1\nR 1\nCX 2 69\nPAULI_CHANNEL_1(0.05,0.1,0.15) 69\nX_ERROR(1.0) 3\nM 2 3\n"
+            "DETECTOR rec[-1] rec[-2]\nOBSERVABLE_INCLUDE(0) rec[-3]\n",
+            "QUBITS 3\nH 0\nCX 0 1\nSQRT_X 2\nMPP X0*Z1*Y2 Z0*Z1\nDEPOLARIZE1(0.3) 1\nM 0 1 2\nMX 0\nMY 1\n"
+            "DETECTOR rec[-6] rec[-5]\nOBSERVABLE_INCLUDE(1) rec[-1]\n"};
+        const uint64_t want[2][3] = {{4, 1, 1}, {7, 1, 2}};
+        for (int i = 0; i < 2; i++) {
+            ncsv_circuit *cc = nullptr;
+            CHECK(ncsv_circuit_parse(ts[i], &cc) == NCSV_OK);
+            ncsv_frames *f = nullptr;
+            CHECK(ncsv_frames_new(cc, 7, &f) == NCSV_OK);
+            uint64_t m = 0, d = 0, o = 0;
+            CHECK(ncsv_frames_shape(f, &m, &d, &o) == NCSV_OK && m == want[i][0] && d == want[i][1] &&
+                  o == want[i][2]);
+            std::vector<uint8_t> rec(m + 1);
+            CHECK(ncsv_frames_reference(f, rec.data(), m) == NCSV_OK);
+            for (uint64_t shots : {1ull, 64ull, 77ull, 300ull}) {
+                std::vector<uint64_t> out(m * ((shots + 63) / 64));
+                CHECK(ncsv_frames_emulate_packed(f, shots, 11, out.data()) == NCSV_OK);
+            }
+            ncsv_frames_destroy(f);
+            ncsv_circuit_destroy(cc);
+        }
+        ncsv_circuit *cc = nullptr;
+        CHECK(ncsv_circuit_parse("QUBITS 1\nT 0\nM 0\n", &cc) == NCSV_OK);
+        ncsv_frames *f = nullptr;
+        CHECK(ncsv_frames_new(cc, 0, &f) == NCSV_UNSUPPORTED && f == nullptr);
+        ncsv_circuit_destroy(cc);
+    }
     // no device: an error, not a crash
     ncsv_state *s = nullptr;
     if (ncsv_device_count() == 0) CHECK(ncsv_create(4, NCSV_C128, &s) == NCSV_DEVICE && s == nullptr);
