@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <tuple>
 
 namespace ncsv {
@@ -520,6 +522,7 @@ struct Plan {
     void *kern_from_basis = nullptr;  // the first item's kernel started from a basis state (made with the plan)
     void *dbuf = nullptr;
     TileGeom geo;  // the tile shape the plan was made for
+    std::vector<jit::MemShape> mem;  // memory layout and block order of each specialized sweep
     ~Plan() {
         for (void *k : kern)
             if (k) jit::release(k);
@@ -529,6 +532,70 @@ struct Plan {
 };
 
 
+
+}  // namespace
+
+// The memory layouts a specialized sweep may load and store in: the default
+// (the top R tile positions), the R positions holding the lowest and the
+// highest qubits, every other position by qubit order (both phases), and the
+// first and last register stage's layouts (an exchange fewer) where they
+// leave the fixed low positions to the threads.
+std::vector<uint32_t> mem_candidates(const SweepH &sw, const TileGeom &geo) {
+    const uint32_t L = geo.L(), R = geo.R, c = geo.c;
+    std::vector<uint32_t> out;
+    auto add = [&](uint32_t m) {
+        if (!m || (m & ((1u << c) - 1)) || (uint32_t)__builtin_popcount(m) != R || (m >> L)) return;
+        if (std::find(out.begin(), out.end(), m) == out.end()) out.push_back(m);
+    };
+    add(jit::default_memr(geo));
+    std::vector<uint32_t> P;
+    for (uint32_t p = c; p < L; p++) P.push_back(p);
+    std::sort(P.begin(), P.end(), [&](uint32_t a, uint32_t b) { return sw.tq[a] < sw.tq[b]; });
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t i = 0; i < R; i++) {
+        lo |= 1u << P[i];
+        hi |= 1u << P[P.size() - 1 - i];
+    }
+    add(lo);
+    add(hi);
+    for (uint32_t ph = 0; ph < 2; ph++) {
+        uint32_t m = 0, k = 0;
+        for (uint32_t i = ph; i < P.size() && k < R; i += 2, k++) m |= 1u << P[i];
+        for (uint32_t i = 1 - ph; i < P.size() && k < R; i += 2, k++) m |= 1u << P[i];
+        add(m);
+    }
+    if (!sw.stages.empty()) {
+        add(sw.stages.front().rmask_pos);
+        add(sw.stages.back().rmask_pos);
+    }
+    return out;
+}
+
+// The first and last register stage's layouts as the load and store layouts
+// where they leave the fixed low positions to the threads (an exchange fewer
+// each), else the default.
+jit::MemShape stage_memory(const SweepH &sw, const TileGeom &geo) {
+    jit::MemShape m;
+    if (sw.stages.empty()) return m;
+    const uint32_t low = (1u << geo.c) - 1;
+    const uint32_t f = sw.stages.front().rmask_pos, l = sw.stages.back().rmask_pos;
+    if (!(f & low) && (uint32_t)__builtin_popcount(f) == geo.R) m.load = f;
+    if (!(l & low) && (uint32_t)__builtin_popcount(l) == geo.R) m.store = l;
+    return m;
+}
+
+jit::MemShape forced_memory(const SweepH &sw, const TileGeom &geo) {
+    jit::MemShape m;
+    const char *a = getenv("NCSV_JIT_MEMSHAPE"), *b = getenv("NCSV_JIT_MEMSHAPE_STORE");
+    if (!a && !b) return stage_memory(sw, geo);
+    if (a && !strcmp(a, "default")) return m;
+    const std::vector<uint32_t> c = mem_candidates(sw, geo);
+    if (a) m.load = m.store = c[(size_t)atoi(a) % c.size()];
+    if (b) m.store = c[(size_t)atoi(b) % c.size()];
+    return m;
+}
+
+namespace {
 
 }  // namespace
 
@@ -694,6 +761,10 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
         size_t total = 0;
         std::vector<char> host;
         plan->kern.assign(plan->items.size(), nullptr);
+        plan->mem.assign(plan->items.size(), jit::MemShape());
+        if (jt)
+            for (size_t k = 0; k < plan->items.size(); k++)
+                if (plan->items[k].tile) plan->mem[k] = forced_memory(plan->items[k].sw, geo);
         if (jt) {
             // specialized kernels: sources compile in parallel; coefficients as T
             const size_t tsz = s->prec == NCSV_C128 ? 8 : 4;
@@ -701,7 +772,8 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
                 plan->prog_off.push_back(total);
                 if (!plan->items[k].tile) continue;
                 std::vector<double> coef;
-                const std::string src = jit::generate(plan->items[k].sw, s->n, geo, s->prec == NCSV_C128, &coef);
+                const std::string src =
+                    jit::generate(plan->items[k].sw, s->n, geo, s->prec == NCSV_C128, &coef, false, &plan->mem[k]);
                 plan->kern[k] = jit::prepare(src);
                 host.resize(total + (coef.size() * tsz + 255) / 256 * 256 + 256);
                 for (size_t t = 0; t < coef.size(); t++) {
@@ -719,7 +791,7 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
             if (!plan->items.empty() && plan->items[0].tile) {
                 std::vector<double> cb;
                 plan->kern_from_basis = jit::prepare_named(
-                    jit::generate(plan->items[0].sw, s->n, geo, s->prec == NCSV_C128, &cb, true),
+                    jit::generate(plan->items[0].sw, s->n, geo, s->prec == NCSV_C128, &cb, true, &plan->mem[0]),
                     "ncsv_sweep_from_basis");
             }
         } else {
@@ -752,7 +824,8 @@ void run_gate_segment(ncsv_state *s, ncsv_circuit *c, size_t i, size_t j) {
                     if (!plan->kern_from_basis) {
                         std::vector<double> cb;
                         plan->kern_from_basis = jit::prepare_named(
-                            jit::generate(it.sw, s->n, geo, s->prec == NCSV_C128, &cb, true), "ncsv_sweep_from_basis");
+                            jit::generate(it.sw, s->n, geo, s->prec == NCSV_C128, &cb, true, &plan->mem[0]),
+                            "ncsv_sweep_from_basis");
                     }
                     kb = plan->kern_from_basis;
                 }

@@ -83,6 +83,19 @@ std::vector<SweepH> plan_lds_sweeps(const std::vector<LOp> &ops, uint32_t n, uin
 // and setting *geo to its shape.
 std::vector<SegItem> plan_segment_tuned(const std::vector<LOp> &ops, uint32_t n, TileGeom *geo, bool specialized);
 
+// The memory layouts (register-position masks) a specialized sweep may load
+// and store in, the default first.
+std::vector<uint32_t> mem_candidates(const SweepH &sw, const TileGeom &geo);
+namespace jit {
+struct MemShape;
+}
+// The memory shape of a specialized sweep: loads in the first register
+// stage's layout and stores in the last one's where those leave the fixed low
+// positions to the threads (an LDS exchange fewer each), else the default
+// (top R positions).  NCSV_JIT_MEMSHAPE=k (and NCSV_JIT_MEMSHAPE_STORE=k)
+// force candidate k of mem_candidates instead (tests), "default" the default.
+jit::MemShape forced_memory(const SweepH &sw, const TileGeom &geo);
+
 // Whether the state runs kernels specialized per plan (fusion 2, or fusion 1
 // from NCSV_JIT_MIN_QUBITS = 20 qubits on; never with NCSV_JIT=0).
 bool use_specialized(const ncsv_state *s);

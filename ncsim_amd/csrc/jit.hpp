@@ -8,12 +8,22 @@
 
 namespace ncsv {
 namespace jit {
+// How a sweep kernel meets memory: the register positions of its loads and
+// of its stores (R-subsets of the tile positions c..L-1; 0 = the top R
+// positions) and the order of its workgroups over the tiles (0 ascending
+// block index, 1 XCD-chunked: XCD x takes a contiguous run of tiles; -1
+// NCSV_JIT_BORDER, default 0).
+struct MemShape {
+    uint32_t load = 0, store = 0;
+    int border = -1;
+};
+uint32_t default_memr(const TileGeom &geo);
 // HIP source of one tile sweep; appends the sweep's coefficients to *coef.
 // from_basis: the sweep starts from the basis state |binit> (a kernel
 // argument) computed in registers instead of loaded, with the same
 // coefficients in the same order.
 std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl, std::vector<double> *coef,
-                     bool from_basis = false);
+                     bool from_basis = false, const MemShape *ms = nullptr);
 // hiprtc compile for `arch`; empty on failure (message in *log).
 std::vector<char> compile_to_code(const std::string &src, const std::string &arch, std::string *log);
 // Starts (or finds) the compile of `src`; returns an opaque handle.

@@ -87,13 +87,15 @@ def host_kernel(src, workdir, name):
     return ctypes.CDLL(so)
 
 
-def run_plan_on_host(text, precision, monkeypatch, half):
+def run_plan_on_host(text, precision, monkeypatch, half, env=None):
     """Every sweep kernel of the circuit's plan, in order, on the host;
     returns (host result, emulated result, plan stats)."""
     work = tempfile.mkdtemp()
     try:
         monkeypatch.setenv("NCSV_JIT_DUMP", work)
         monkeypatch.setenv("NCSV_JIT_HALF", str(half))
+        for k, v in (env or {}).items():
+            monkeypatch.setenv(k, v)
         c = H.Circuit.parse(text)
         n = c.num_qubits()
         k = ctypes.c_uint32()
@@ -157,3 +159,51 @@ def test_generated_sweeps_on_host_threads(name, precision, make, half, monkeypat
     got, want, stats = run_plan_on_host(make(), precision, monkeypatch, half)
     tol = 1e-13 if precision == H.C128 else 2e-6
     assert np.abs(got - want).max() < tol, (name, stats)
+
+
+# memory shapes: loads and stores in other register layouts (the candidates
+# the layout timing chooses from, including different load and store
+# layouts) and the XCD-chunked block order; every shape computes the same
+# state.  The block order is a compile-time constant, so each case runs in a
+# fresh process.
+SHAPES = [
+    ("outer-load2", H.C128, lambda: outer_heavy(21, 12, 4), {"NCSV_JIT_MEMSHAPE": "2"}),
+    ("outer-load3-store5-border", H.C128, lambda: outer_heavy(21, 12, 4),
+     {"NCSV_JIT_MEMSHAPE": "3", "NCSV_JIT_MEMSHAPE_STORE": "5", "NCSV_JIT_BORDER": "1"}),
+    ("brickwork-c64-first-last", H.C64, lambda: W.rotation_brickwork(17, 4, 3),
+     {"NCSV_JIT_MEMSHAPE": "5", "NCSV_JIT_MEMSHAPE_STORE": "6", "NCSV_JIT_BORDER": "1"}),
+    ("qft-c64-load1-border", H.C64, lambda: W.qft_with_rz(17, 12, 2), {"NCSV_JIT_MEMSHAPE": "1", "NCSV_JIT_BORDER": "1"}),
+    ("brickwork-load4-store1", H.C128, lambda: W.rotation_brickwork(16, 4, 6),
+     {"NCSV_JIT_MEMSHAPE": "4", "NCSV_JIT_MEMSHAPE_STORE": "1"}),
+]
+
+
+def _shape_worker(name, q):
+    try:
+        case = {c[0]: c for c in SHAPES}[name]
+        _, precision, make, env = case
+
+        class MP:  # a minimal monkeypatch for the child process
+            def setenv(self, k, v):
+                os.environ[k] = v
+
+        got, want, stats = run_plan_on_host(make(), precision, MP(), 1, env)
+        q.put(float(np.abs(got - want).max()))
+    except BaseException as e:  # noqa: BLE001
+        q.put(repr(e))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ with C++20")
+@pytest.mark.parametrize("name", [c[0] for c in SHAPES])
+def test_generated_sweeps_in_other_memory_shapes(name):
+    import multiprocessing as mp
+
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    p = ctx.Process(target=_shape_worker, args=(name, q))
+    p.start()
+    r = q.get(timeout=600)
+    p.join(60)
+    assert not isinstance(r, str), r
+    precision = {c[0]: c[1] for c in SHAPES}[name]
+    assert r < (1e-13 if precision == H.C128 else 2e-6), (name, r)
