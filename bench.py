@@ -449,6 +449,52 @@ def run_frames(args):
                                  "detectors": f.num_detectors, "shots": shots}}), flush=True)
 
 
+def run_su4(args):
+    """A random two-qubit circuit: a line of Haar-random 4x4 unitaries on
+    alternating neighbour pairs, `--depth` layers on `--qubits` qubits,
+    complex64 by default; one step = set_basis(0) + the circuit.  Reports
+    the dominant kernel's roofline.  The sweeps' FP32 MFMA blocks follow
+    NCSV_JIT_MFMA (0, the default: none; 1: by cost)."""
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+    n, depth = args.qubits, args.depth
+    prec = H.C128 if args.precision == "c128" else H.C64
+    gates = W.su4_brickwork(n, depth, 3)
+    circ = W.build_circuit(n, gates, H.Circuit, H.Gate)
+    s = H.HipStateVector.zero_state(n, prec)
+    s.set_fusion(args.fusion)
+    for _ in range(max(1, args.warmup)):
+        s.set_basis(0)
+        s.run(circ)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s.set_basis(0)
+        s.run(circ)
+    s.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    s.set_basis(0)
+    s.profile_reset()
+    s.profile(True)
+    s.run(circ)
+    s.profile(False)
+    st = s.profile_read()
+    name, (launches, tot_ms, byts) = max(st.items(), key=lambda kv: kv[1][1])
+    achieved = byts / launches / (tot_ms / launches / 1000.0)
+    log("su4: kernels " + ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(st.items())))
+    print(json.dumps({"metric": "amp-updates/s", "value": len(gates) * float(1 << n) / (ms / 1000.0),
+                      "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": max(1, args.warmup),
+                      "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "c64 (f32, f64 reductions)" if prec == H.C64 else "c128 (f64)",
+                      "data": "synthetic (seeded circuit)",
+                      "config": {"workload": f"random SU(4) brickwork {n} qubits depth {depth}", "gates": len(gates),
+                                 "mfma_blocks": os.environ.get("NCSV_JIT_MFMA", "0")},
+                      "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
+                                   "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                                   "launches": launches, "avg_ms": tot_ms / launches,
+                                   "sweeps_per_step": s.sweep_count()}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,7 +507,7 @@ def main():
     ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--only-n", action="store_true", help="qft workload: only the n-qubit complex64 runs")
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "qft", "batch", "trajectories", "frames", "copy"])
+                    choices=["brickwork", "qft", "batch", "trajectories", "frames", "copy", "su4"])
     ap.add_argument("--distance", type=int, default=25, help="frames workload: repetition-code distance")
     ap.add_argument("--rounds", type=int, default=25, help="frames workload: rounds")
     ap.add_argument("--shots", type=int, default=1 << 24, help="frames workload: shots per step")
@@ -493,6 +539,9 @@ def main():
         return
     if args.workload == "copy":
         run_copy(args)
+        return
+    if args.workload == "su4":
+        run_su4(args)
         return
 
     import ncsim_amd as H

@@ -223,7 +223,16 @@ uint64_t choose_set(const std::vector<Cls> &cl, const std::vector<uint32_t> &rem
                 best = (int)q;
             }
         }
-        if (best < 0) best = (int)cands[0];  // no single qubit helps: take the first needed
+        if (best < 0) {
+            // no single qubit helps: take the first needed (in need order, not
+            // the variant's permuted order)
+            best = (int)cands[0];
+            for (uint32_t q : order)
+                if ((cand >> q) & 1) {
+                    best = (int)q;
+                    break;
+                }
+        }
         T |= 1ull << best;
         pick(cl, rem, T, nullptr, &cur);
     }
@@ -695,7 +704,12 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
         if (n >= 20) {
             auto bs = score(best);
             for (uint32_t v = 1; v < 24; v++) {
-                std::vector<SweepH> c = plan_sweeps(cl, run, n, geo, 0, v);
+                std::vector<SweepH> c;
+                try {
+                    c = plan_sweeps(cl, run, n, geo, 0, v);
+                } catch (const Error &) {
+                    continue;  // a variant that stalls is only a worse choice
+                }
                 const auto cs = score(c);
                 if (cs < bs) {
                     best.swap(c);

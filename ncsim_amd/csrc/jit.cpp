@@ -42,6 +42,7 @@ struct Gen {
     double scale = 1.0;  // common factor taken out of the real-form 2x2s, applied at the store
     bool opaque = false;  // exchange addresses recomputed where used (kernels that loop over tiles)
     int S = 4;            // LDS swizzle width (16-byte slots: 4; 8-byte: 5)
+    int mfma_blocks = 0;  // FP32 MFMA dense blocks emitted
     int half = 0;         // exchanges through half a tile of LDS, in two passes (2, 3: test the
                           // wavefront-turn and intermediate-layout forms wherever they apply)
 
@@ -861,6 +862,238 @@ uint32_t swap_bits(uint32_t m, uint32_t a, uint32_t b) {
     return m ^ ((1u << a) | (1u << b));
 }
 
+// ---------------------------------------------- FP32 MFMA dense blocks (complex64)
+//
+// A run of a stage's gate ops on at most four register bits becomes one
+// 16x16 complex unitary U, composed on the host in FP64, and is applied with
+// v_mfma_f32_32x32x2f32 as the real 32x32 form [[Re U, -Im U], [Im U, Re U]]:
+// one block register bit b3 is first exchanged with lane bit 5 (one
+// v_permlane32_swap per register pair and component), so each 32-lane column
+// n = lane & 31 holds a 16-amplitude vector as 32 reals, 16 per lane half
+// (slot s = component + 2 * the bits b0..b2, half h = the bit b3).  The
+// instruction's B operand j is slot j of each lane (K = 2j + h), its D
+// register i is the output row 8(i>>2) + 4h + (i&3), so with A's rows and
+// columns ordered accordingly every lane gets back the slots it supplied; the
+// four groups of the other two register bits take 16 instructions each.  A
+// lane's A operand j is A[lane & 31][2j + (lane >> 5)], read from the
+// coefficient buffer (16 coalesced loads per block).  Lane layouts checked on
+// MI355X (gfx950): D register i, lane l = D[8(i/4) + 4(l/32) + i%4][l%32];
+// permlane32_swap(x, y) returns (x lanes 0-31 | y lanes 0-31) and
+// (x lanes 32-63 | y lanes 32-63).
+//
+// At the rated 64 cycles per instruction a block's 16 x 4 MFMAs per wavefront
+// take the time of ~32 FP32 vector instructions per amplitude, so runs whose
+// ops cost more (NCSV_JIT_MFMA_COST, default 36: three generic 4x4s) would
+// gain.  Measured on MI355X they do not: on a 30-qubit random SU(4)
+// brickwork (complex64, 25 sweeps, blocks of three 4x4s) the sweeps take
+// 6.37 ms with blocks against 4.87 ms without (blocks of four 4x4s: 4.92
+// against 4.87 ms), so blocks are off by default.  NCSV_JIT_MFMA: 0 never
+// (default), 1 by the cost above, 2 for every run with arithmetic.
+int mfma_mode() {
+    const char *e = getenv("NCSV_JIT_MFMA");
+    return e ? atoi(e) : 0;
+}
+
+const char *MFMA_HELPERS =
+    "typedef float f16v __attribute__((ext_vector_type(16)));  // @mfma\n"
+    "__device__ __forceinline__ f16v mfma32(float a, float b, f16v c) { return "
+    "__builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0); }  // @mfma\n"
+    "__device__ __forceinline__ void swap32(float &x, float &y) { auto r = "
+    "__builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false); "
+    "x = __uint_as_float(r[0]); y = __uint_as_float(r[1]); }  // @mfma\n";
+
+bool mfma_absorbable(const TileOpH &op) {
+    switch (op.kind) {
+    case TK_U1R:
+    case TK_U1X:
+    case TK_U1:
+    case TK_U2:
+    case TK_SWAPR: return true;
+    case TK_PERM1:
+    case TK_DM: return op.gmask == 0;
+    default: return false;
+    }
+}
+
+uint32_t mfma_op_bits(const TileOpH &op) {
+    switch (op.kind) {
+    case TK_U1R:
+    case TK_U1X:
+    case TK_U1: return 1u << op.j0;
+    case TK_U2:
+    case TK_SWAPR: return (1u << op.j0) | (1u << op.j1);
+    case TK_PERM1: return (1u << op.j0) | op.rmask;
+    default: return op.rmask;
+    }
+}
+
+// Vector instructions per amplitude an op costs in the register sweeps.
+double valu_cost(const TileOpH &op) {
+    switch (op.kind) {
+    case TK_U1R:
+    case TK_U1X: return 2.0;
+    case TK_U1: return 8.0;
+    case TK_U2: return 16.0;
+    case TK_PERM1: return (op.k0 == 0 && op.k1 == 0 && op.rmask == 0) ? 0.0 : 1.0;
+    case TK_SWAPR: return 0.0;
+    default: return 0.5;
+    }
+}
+double mfma_block_cost() {
+    const char *e = getenv("NCSV_JIT_MFMA_COST");
+    return e ? atof(e) : 36.0;
+}
+
+// U (16x16, U[row * 16 + col]) of the ops on register bits bb[0..3] (local
+// index bit k = register bit bb[k]), composed in FP64 in program order.
+void mfma_block_matrix(const std::vector<const TileOpH *> &ops, const uint32_t bb[4], cd *U) {
+    auto lb = [&](uint32_t j) {
+        for (int k = 0; k < 4; k++)
+            if (bb[k] == j) return (uint32_t)k;
+        fail(NCSV_INVALID, "jit: MFMA block op outside the block");
+        return 0u;
+    };
+    auto lmask = [&](uint32_t rmask) {
+        uint32_t m = 0;
+        for (uint32_t a = rmask; a; a &= a - 1) m |= 1u << lb((uint32_t)__builtin_ctz(a));
+        return m;
+    };
+    auto ip = [](int k, cd a) {
+        switch (k & 3) {
+        case 0: return a;
+        case 1: return cd(-a.imag(), a.real());
+        case 2: return -a;
+        default: return cd(a.imag(), -a.real());
+        }
+    };
+    for (uint32_t col = 0; col < 16; col++) {
+        cd v[16];
+        for (uint32_t a = 0; a < 16; a++) v[a] = a == col ? cd(1.0, 0.0) : cd(0.0, 0.0);
+        for (const TileOpH *o : ops) {
+            const TileOpH &op = *o;
+            switch (op.kind) {
+            case TK_U1R:
+            case TK_U1X:
+            case TK_U1: {
+                const uint32_t J = 1u << lb(op.j0);
+                for (uint32_t a = 0; a < 16; a++) {
+                    if (a & J) continue;
+                    const cd x = v[a], y = v[a | J];
+                    v[a] = op.m[0] * x + op.m[1] * y;
+                    v[a | J] = op.m[2] * x + op.m[3] * y;
+                }
+                break;
+            }
+            case TK_U2: {
+                const uint32_t A = 1u << lb(op.j0), B = 1u << lb(op.j1);
+                for (uint32_t a = 0; a < 16; a++) {
+                    if (a & (A | B)) continue;
+                    const uint32_t ix[4] = {a, a | A, a | B, a | A | B};
+                    cd x[4], y[4];
+                    for (int t = 0; t < 4; t++) x[t] = v[ix[t]];
+                    for (int t = 0; t < 4; t++) {
+                        y[t] = 0.0;
+                        for (int u = 0; u < 4; u++) y[t] += op.m[4 * t + u] * x[u];
+                    }
+                    for (int t = 0; t < 4; t++) v[ix[t]] = y[t];
+                }
+                break;
+            }
+            case TK_PERM1: {
+                const uint32_t J = 1u << lb(op.j0), cm = lmask(op.rmask);
+                for (uint32_t a = 0; a < 16; a++) {
+                    if ((a & J) || (a & cm) != cm) continue;
+                    const cd x = v[a], y = v[a | J];
+                    v[a] = ip(op.k1, y);
+                    v[a | J] = ip(op.k0, x);
+                }
+                break;
+            }
+            case TK_SWAPR: {
+                const uint32_t A = 1u << lb(op.j0), B = 1u << lb(op.j1);
+                for (uint32_t a = 0; a < 16; a++)
+                    if ((a & A) && !(a & B)) std::swap(v[a], v[(a & ~A) | B]);
+                break;
+            }
+            case TK_DM: {
+                const uint32_t cm = lmask(op.rmask);
+                for (uint32_t a = 0; a < 16; a++)
+                    if ((a & cm) == cm) v[a] *= op.m[0];
+                break;
+            }
+            default: fail(NCSV_INVALID, "jit: op kind not composable into an MFMA block");
+            }
+        }
+        for (uint32_t r = 0; r < 16; r++) U[r * 16 + col] = v[r];
+    }
+}
+
+// Emits the block of `ops` on register bits `bits` (at most four; padded with
+// other register bits).  Complex64 tiles with 32 registers per thread only.
+void emit_mfma_block(Gen &g, const std::vector<const TileOpH *> &ops, uint32_t bits) {
+    auto &o = g.o;
+    uint32_t bb[4], nb = 0;
+    for (uint32_t j = 0; j < g.R && nb < 4; j++)
+        if ((bits >> j) & 1) bb[nb++] = j;
+    for (uint32_t j = 0; j < g.R && nb < 4; j++)
+        if (!((bits >> j) & 1)) bits |= 1u << (bb[nb++] = j);
+    std::sort(bb, bb + 4);
+    uint32_t u = 0;
+    for (uint32_t j = 0; j < g.R; j++)
+        if (!((bits >> j) & 1)) u = j;
+    const uint32_t bs = bb[3];
+    cd U[256];
+    mfma_block_matrix(ops, bb, U);
+    // register of (group gi over bits u, bs) and block index t over b0..b2
+    auto reg = [&](uint32_t gi, uint32_t t) {
+        uint32_t r = 0;
+        for (int k = 0; k < 3; k++)
+            if ((t >> k) & 1) r |= 1u << bb[k];
+        if (gi & 1) r |= 1u << u;
+        if (gi & 2) r |= 1u << bs;
+        return r;
+    };
+    // real element (h, s): amplitude a = (s >> 1) | (h << 3), component s & 1
+    auto ureal = [&](uint32_t h, uint32_t s, uint32_t h2, uint32_t s2) {
+        const cd z = U[(((s >> 1) | (h << 3)) * 16) + ((s2 >> 1) | (h2 << 3))];
+        const uint32_t c = s & 1, c2 = s2 & 1;
+        if (c == 0 && c2 == 0) return z.real();
+        if (c == 0 && c2 == 1) return -z.imag();
+        if (c == 1 && c2 == 0) return z.imag();
+        return z.real();
+    };
+    while (g.coef->size() % 64) g.push(0.0);
+    const uint32_t off = (uint32_t)g.coef->size();
+    for (uint32_t j = 0; j < 16; j++)
+        for (uint32_t l = 0; l < 64; l++) {
+            const uint32_t M = l & 31, h = (M >> 2) & 1, s = (M & 3) | ((M >> 3) << 2);
+            g.push(ureal(h, s, l >> 5, j));
+        }
+    auto swap_bs = [&]() {
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if ((r >> bs) & 1) continue;
+            const std::string a = g.var(r), b = g.var(r | (1u << bs));
+            o << "    swap32(" << a << ".x, " << b << ".x); swap32(" << a << ".y, " << b << ".y);\n";
+        }
+    };
+    o << "  { // MFMA block on register bits " << bb[0] << " " << bb[1] << " " << bb[2] << " " << bb[3] << " (" << ops.size()
+      << " ops)\n";
+    swap_bs();
+    o << "    const unsigned ln = tid & 63u;\n";
+    for (uint32_t j = 0; j < 16; j++) o << "    const float A" << j << " = c[" << off + 64 * j << "u + ln];\n";
+    for (uint32_t gi = 0; gi < 4; gi++) {
+        o << "    { f16v m = {};\n";
+        for (uint32_t j = 0; j < 16; j++)
+            o << "      m = mfma32(A" << j << ", " << g.var(reg(gi, j >> 1)) << ((j & 1) ? ".y" : ".x") << ", m);\n";
+        for (uint32_t i = 0; i < 16; i++)
+            o << "      " << g.var(reg(gi, i >> 1)) << ((i & 1) ? ".y" : ".x") << " = m[" << i << "];\n";
+        o << "    }\n";
+    }
+    swap_bs();
+    o << "  }\n";
+    g.mfma_blocks++;
+}
+
 void emit_stages_lazy(Gen &g, const SweepH &sw, uint32_t &cur, uint32_t memr) {
     auto &o = g.o;
     std::map<uint32_t, Pending> pend;
@@ -881,8 +1114,7 @@ void emit_stages_lazy(Gen &g, const SweepH &sw, uint32_t &cur, uint32_t memr) {
             emit_exchange(g, cur, St.rmask_pos);
             cur = St.rmask_pos;
         }
-        for (uint32_t k = St.op_begin; k < St.op_end; k++) {
-            const TileOpH &op = sw.ops[k];
+        auto process = [&](const TileOpH &op) {
             switch (op.kind) {
             case TK_DM: {
                 Pending &p = pend[op.rmask];
@@ -916,7 +1148,55 @@ void emit_stages_lazy(Gen &g, const SweepH &sw, uint32_t &cur, uint32_t memr) {
             case TK_U2: flush((1u << op.j0) | (1u << op.j1), false); emit_op(g, op, cur); break;
             default: flush(1u << op.j0, false); emit_op(g, op, cur); break;
             }
+        };
+        // runs of ops on at most four register bits, and the conditioned
+        // phases on other register bits met inside them (those commute with
+        // the run), as MFMA blocks where that is cheaper
+        const bool mf = !g.dbl && g.R == 5 && mfma_mode() > 0;
+        std::vector<const TileOpH *> run;
+        uint32_t ubits = 0, forbid = 0;
+        double vcost = 0.0;
+        auto close_run = [&]() {
+            if (run.empty()) return;
+            const bool use = mfma_mode() == 2 ? vcost > 0.0 : vcost > mfma_block_cost();
+            if (use) {
+                std::vector<const TileOpH *> blk;
+                for (const TileOpH *op : run) {
+                    if (mfma_absorbable(*op))
+                        blk.push_back(op);
+                    else
+                        process(*op);  // a conditioned phase off the block's bits: pending
+                }
+                flush(ubits, false);
+                emit_mfma_block(g, blk, ubits);
+            } else {
+                for (const TileOpH *op : run) process(*op);
+            }
+            run.clear();
+            ubits = forbid = 0;
+            vcost = 0.0;
+        };
+        for (uint32_t k = St.op_begin; k < St.op_end; k++) {
+            const TileOpH &op = sw.ops[k];
+            if (mf) {
+                if (mfma_absorbable(op)) {
+                    const uint32_t b = mfma_op_bits(op);
+                    if ((b & forbid) || __builtin_popcount(ubits | b) > 4) close_run();
+                    run.push_back(&op);
+                    ubits |= b;
+                    vcost += valu_cost(op);
+                    continue;
+                }
+                if (op.kind == TK_DM && op.gmask && !(op.rmask & ubits) && !run.empty()) {
+                    run.push_back(&op);
+                    forbid |= op.rmask;
+                    continue;
+                }
+                close_run();
+            }
+            process(op);
         }
+        close_run();
     }
     flush(0, true);
     if (cur != memr) {
@@ -987,6 +1267,8 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     if (nt) o << NT_HELPERS;
     o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
       << ")) & " << ((1u << S) - 1) << "u); }\n";
+    const std::string header = o.str();  // the MFMA helpers go here if the body uses them
+    o.str("");
     o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT);
     if (waves) o << ", " << waves;
     o << ") void " << (from_basis ? "ncsv_sweep_from_basis" : "ncsv_sweep")
@@ -1032,6 +1314,10 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
         else
             o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
     }
+    // kernels with MFMA blocks keep every tile load ahead of the arithmetic
+    // (the scheduler otherwise sinks loads into the MFMA chains and waits on
+    // each; the vector-only kernels issue them all first by themselves)
+    o << "/*@loads-issued@*/\n";
     uint32_t cur = memr;
     if (use_lazy_diagonals()) {
         emit_stages_lazy(g, sw, cur, mems);
@@ -1085,6 +1371,17 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
             o << "  amp[gs | " << roff_of(rps, r) << "ull] = " << g.var(r) << ";\n";
     }
     o << "}\n";
+    {
+        std::string body = o.str();
+        const std::string mark = "/*@loads-issued@*/\n";
+        const size_t at = body.find(mark);
+        if (at != std::string::npos)
+            body.replace(at, mark.size(), g.mfma_blocks ? "  __builtin_amdgcn_sched_barrier(0);\n" : "");
+        o.str("");
+        o << header;
+        if (g.mfma_blocks) o << MFMA_HELPERS;
+        o << body;
+    }
     if (const char *dir = getenv("NCSV_JIT_DUMP")) {
         // the sweep's coefficients beside its dumped source (doubles, in order)
         const std::string path =

@@ -166,6 +166,39 @@ def noisy_brickwork(n, depth, seed, p=1e-3, every=10, mqubits=None):
     return "\n".join(lines) + "\n"
 
 
+def su4_brickwork(n, depth, seed):
+    """Random two-qubit circuit on a line (quantum-volume style): per layer a
+    Haar-random 4x4 unitary on each pair (2i, 2i+1) in even layers and
+    (2i+1, 2i+2) in odd ones (complex Gaussian matrix from SplitMix64 draws by
+    Box-Muller, QR, phases of R's diagonal divided out).  Returns a list of
+    (qubits, matrix); bit k of the matrix index is qubits[k]."""
+    import numpy as np
+
+    r = SplitMix64(seed)
+
+    def gauss():
+        u1, u2 = 1.0 - r.gen_f64(), r.gen_f64()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+    out = []
+    for layer in range(depth):
+        for a in range(layer % 2, n - 1, 2):
+            z = np.array([[complex(gauss(), gauss()) for _ in range(4)] for _ in range(4)]) / math.sqrt(2.0)
+            q, rr = np.linalg.qr(z)
+            d = np.diagonal(rr)
+            out.append(((a, a + 1), q * (d / np.abs(d))))
+    return out
+
+
+def build_circuit(n, gates, circuit_cls, gate_cls):
+    """A circuit of (qubits, matrix) UNITARY gates through an API with
+    `circuit_cls(n)` and `.gate(gate_cls("UNITARY", matrix=m), qubits)`."""
+    c = circuit_cls(n)
+    for qs, m in gates:
+        c.gate(gate_cls("UNITARY", matrix=m), list(qs))
+    return c
+
+
 def repetition_code(d, rounds, p):
     """The reference's repetition-code memory experiment (frame.rs:401-442):
     data qubits 0..d-1, ancilla d + a measures Z_a Z_{a+1} with two CXs and

@@ -42,12 +42,43 @@ static std::barrier<> *g_bar;
 #define __restrict__
 #define __shared__ static
 #define __popcll __builtin_popcountll
+#define __builtin_amdgcn_sched_barrier(x)
 inline int __double2hiint(double d) { long long b; memcpy(&b, &d, 8); return (int)(b >> 32); }
 inline int __double2loint(double d) { long long b; memcpy(&b, &d, 8); return (int)(unsigned)(b & 0xffffffffu); }
 inline double __hiloint2double(int hi, int lo) {
     long long b = ((long long)hi << 32) | (unsigned)lo; double d; memcpy(&d, &b, 8); return d; }
 inline int __float_as_int(float f) { int i; memcpy(&i, &f, 4); return i; }
 inline float __int_as_float(int i) { float f; memcpy(&f, &i, 4); return f; }
+// the wavefront operations of the MFMA blocks on host threads (lane = tid & 63,
+// wavefront = tid >> 6); every thread of the workgroup makes the same calls in
+// the same order, so the workgroup barrier orders each exchange
+struct f16v { float v[16]; float &operator[](int i) { return v[i]; } };
+static float g_wa[4][64], g_wb[4][64];
+inline f16v mfma32(float a, float b, f16v c) {
+    // v_mfma_f32_32x32x2f32: A[m][k] in lane m + 32k, B[k][n] in lane n + 32k,
+    // D register i of lane l = row 8(i/4) + 4(l/32) + i%4, column l%32
+    const unsigned w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    g_wa[w][l] = a;
+    g_wb[w][l] = b;
+    g_bar->arrive_and_wait();
+    f16v d = c;
+    for (int i = 0; i < 16; i++) {
+        const unsigned M = 8 * (i / 4) + 4 * (l / 32) + i % 4, n = l % 32;
+        d[i] = g_wa[w][M] * g_wb[w][n] + g_wa[w][M + 32] * g_wb[w][n + 32] + d[i];
+    }
+    g_bar->arrive_and_wait();
+    return d;
+}
+inline void swap32(float &x, float &y) {
+    // v_permlane32_swap: (x lanes 0-31 | y lanes 0-31), (x lanes 32-63 | y lanes 32-63)
+    const unsigned w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    g_wa[w][l] = x;
+    g_wb[w][l] = y;
+    g_bar->arrive_and_wait();
+    x = l < 32 ? g_wa[w][l] : g_wb[w][l - 32];
+    y = l < 32 ? g_wa[w][l + 32] : g_wb[w][l];
+    g_bar->arrive_and_wait();
+}
 '''
 
 POST = r'''
@@ -70,8 +101,9 @@ HOST_NT = "inline V ld_nt(const V *p) { return *p; }\ninline void st_nt(V *p, V 
 def host_kernel(src, workdir, name):
     lines = []
     for line in src.splitlines():
-        if "asm volatile" in line or "ext_vector_type" in line or "__builtin_nontemporal" in line:
-            continue  # the nontemporal helpers become plain accesses below
+        if ("asm volatile" in line or "ext_vector_type" in line or "__builtin_nontemporal" in line
+                or "// @mfma" in line):
+            continue  # the nontemporal and MFMA helpers become the host versions
         lines.append(line)
         if line.startswith("typedef") and line.endswith(" V;") and "ld_nt(" in src:
             lines.append(HOST_NT)
@@ -96,7 +128,7 @@ def run_plan_on_host(text, precision, monkeypatch, half, env=None):
         monkeypatch.setenv("NCSV_JIT_HALF", str(half))
         for k, v in (env or {}).items():
             monkeypatch.setenv(k, v)
-        c = H.Circuit.parse(text)
+        c = text if isinstance(text, H.Circuit) else H.Circuit.parse(text)
         n = c.num_qubits()
         k = ctypes.c_uint32()
         assert L.lib().ncsv_jit_check(c._h, int(precision), b"gfx950", ctypes.byref(k)) == 0
@@ -207,3 +239,87 @@ def test_generated_sweeps_in_other_memory_shapes(name):
     assert not isinstance(r, str), r
     precision = {c[0]: c[1] for c in SHAPES}[name]
     assert r < (1e-13 if precision == H.C128 else 2e-6), (name, r)
+
+
+# FP32 MFMA dense blocks in complex64 sweeps: a random two-qubit circuit takes
+# them by cost; forcing them (NCSV_JIT_MFMA=2) on the rotation brickwork and
+# the QFT composes real-form 2x2s, controlled flips, swaps and phases into
+# blocks.  The kernels' v_mfma_f32_32x32x2f32 and v_permlane32_swap run as
+# wavefront exchanges on the host threads.
+def su4_circuit(n, depth, seed):
+    return W.build_circuit(n, W.su4_brickwork(n, depth, seed), H.Circuit, H.Gate)
+
+
+MFMA_CASES = [
+    ("su4-auto", lambda: su4_circuit(15, 6, 1), {"NCSV_JIT_MFMA": "1"}),
+    ("su4-multi-tile", lambda: su4_circuit(17, 5, 2), {"NCSV_JIT_MFMA": "2"}),
+    ("brickwork-forced", lambda: W.rotation_brickwork(15, 4, 3), {"NCSV_JIT_MFMA": "2"}),
+    ("qft-forced", lambda: W.qft_with_rz(15, 12, 2), {"NCSV_JIT_MFMA": "2"}),
+    ("outer-forced", lambda: outer_heavy(15, 13, 5), {"NCSV_JIT_MFMA": "2"}),
+]
+
+
+def _mfma_worker(name, q):
+    try:
+        _, make, env = {c[0]: c for c in MFMA_CASES}[name]
+        os.environ.update(env)
+
+        class MP:
+            def setenv(self, k, v):
+                os.environ[k] = v
+
+        work = tempfile.mkdtemp()
+        os.environ["NCSV_JIT_DUMP"] = work
+        c = make()
+        got, want, stats = run_plan_on_host(c, H.C64, MP(), 1)
+        q.put(float(np.abs(got - want).max()))
+    except BaseException as e:  # noqa: BLE001
+        import traceback
+        q.put(traceback.format_exc())
+
+
+def _mfma_blocks_in(circuit_or_text, env):
+    """MFMA blocks the generator writes for the circuit's complex64 plan."""
+    import subprocess as sp
+    import sys
+    code = (
+        "import os, ctypes, glob, sys\n"
+        "import ncsim_amd as H\nfrom ncsim_amd import _lib as L, workloads as W\n"
+        "c = W.build_circuit(15, W.su4_brickwork(15, 6, 1), H.Circuit, H.Gate) if sys.argv[1] == 'su4' "
+        "else H.Circuit.parse(W.rotation_brickwork(15, 4, 3))\n"
+        "k = ctypes.c_uint32()\n"
+        "assert L.lib().ncsv_jit_check(c._h, int(H.C64), b'gfx950', ctypes.byref(k)) == 0\n"
+        "print(sum(open(f).read().count('MFMA block') for f in glob.glob(os.environ['NCSV_JIT_DUMP'] + '/*.hip')))\n")
+    work = tempfile.mkdtemp()
+    try:
+        e = dict(os.environ, NCSV_JIT_DUMP=work, **env)
+        r = sp.run([sys.executable, "-c", code, circuit_or_text], env=e, capture_output=True, text=True,
+                   cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        assert r.returncode == 0, r.stderr[-2000:]
+        return int(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def test_mfma_blocks_chosen_by_cost():
+    # by cost: generic 4x4s get blocks, rotations (two instructions per
+    # amplitude each) none; off by default; forced on every run
+    assert _mfma_blocks_in("su4", {"NCSV_JIT_MFMA": "1"}) > 0
+    assert _mfma_blocks_in("brickwork", {"NCSV_JIT_MFMA": "1"}) == 0
+    assert _mfma_blocks_in("su4", {}) == 0
+    assert _mfma_blocks_in("brickwork", {"NCSV_JIT_MFMA": "2"}) > 0
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ with C++20")
+@pytest.mark.parametrize("name", [c[0] for c in MFMA_CASES])
+def test_mfma_blocks_on_host_threads(name):
+    import multiprocessing as mp
+
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    p = ctx.Process(target=_mfma_worker, args=(name, q))
+    p.start()
+    r = q.get(timeout=900)
+    p.join(60)
+    assert not isinstance(r, str), r
+    assert r < 2e-6, (name, r)
