@@ -43,6 +43,7 @@ struct Gen {
     bool opaque = false;  // exchange addresses recomputed where used (kernels that loop over tiles)
     int S = 4;            // LDS swizzle width (16-byte slots: 4; 8-byte: 5)
     int mfma_blocks = 0;  // FP32 MFMA dense blocks emitted
+    const char *bar = "__syncthreads()";  // the exchanges' workgroup barrier
     int half = 0;         // exchanges through half a tile of LDS, in two passes (2, 3: test the
                           // wavefront-turn and intermediate-layout forms wherever they apply)
 
@@ -253,11 +254,11 @@ void emit_exchange_wave_halves(Gen &g, uint32_t from, uint32_t to, int i) {
         o << "    if (((tid >> " << i << ") & 1u) == " << h << "u) {\n";
         for (uint32_t r = 0; r < g.NR; r++)
             o << "      lds[" << lidx(g.S, "sa", ma, roffs(ra, r)) << "] = " << g.var(r) << ";\n";
-        o << "    }\n    __syncthreads();\n";
+        o << "    }\n    " << g.bar << ";\n";
         o << "    if (((tid >> " << i << ") & 1u) == " << h << "u) {\n";
         for (uint32_t r = 0; r < g.NR; r++)
             o << "      " << g.var(r) << " = lds[" << lidx(g.S, "sb", mb, roffs(rb, r)) << "];\n";
-        o << "    }\n    __syncthreads();\n";
+        o << "    }\n    " << g.bar << ";\n";
     }
     o << "  }\n";
 }
@@ -318,14 +319,14 @@ void emit_exchange_half(Gen &g, uint32_t from, uint32_t to) {
             o << "    lds[" << lidx(g.S, "sa", ma, roffs(ra, ja, r)) << "] = " << g.var(r) << ";\n";
             freed.push_back(g.name[r]);
         }
-        o << "    __syncthreads();\n";
+        o << "    " << g.bar << ";\n";
         size_t k = 0;
         for (uint32_t r = 0; r < g.NR; r++) {
             if (((r >> jb) & 1u) != h) continue;
             renamed[r] = freed[k++];
             o << "    a" << renamed[r] << " = lds[" << lidx(g.S, "sb", mb, roffs(rb, jb, r)) << "];\n";
         }
-        o << "    __syncthreads();\n";
+        o << "    " << g.bar << ";\n";
     }
     o << "  }\n";
     g.name = renamed;
@@ -348,7 +349,7 @@ void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
             if ((r >> j) & 1) off |= 1u << ra[j];
         o << "    lds[" << lidx(g.S, "sa", bits_of(ta), off) << "] = " << g.var(r) << ";\n";
     }
-    o << "    __syncthreads();\n    unsigned tb = " << deposit("tid", tb, false) << ";\n";
+    o << "    " << g.bar << ";\n    unsigned tb = " << deposit("tid", tb, false) << ";\n";
     if (g.opaque) o << "    asm volatile(\"\" : \"+v\"(tb));\n";
     o << "    const unsigned sb = swz(tb);\n";
     for (uint32_t r = 0; r < g.NR; r++) {
@@ -357,7 +358,7 @@ void emit_exchange(Gen &g, uint32_t from, uint32_t to) {
             if ((r >> j) & 1) off |= 1u << rb[j];
         o << "    " << g.var(r) << " = lds[" << lidx(g.S, "sb", bits_of(tb), off) << "];\n";
     }
-    o << "    __syncthreads();\n  }\n";
+    o << "    " << g.bar << ";\n  }\n";
 }
 
 bool is_exact(cd c, double re, double im) { return c.real() == re && c.imag() == im; }
@@ -1407,6 +1408,16 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
 // term(c^x) = (-1)^pt i^(ph + 2 pr(r^xr)) conj(P), pt the Z parity of the
 // thread and outer bits, pr that of the register bits.  One workgroup walks
 // tiles with a grid stride; partial sums go to part[k * gridDim.x + block].
+// Complex128 expectation kernels load each tile's successor while working
+// on it (30 qubits, 64 Paulis: 3.149 -> 3.072 ms per pass on MI355X); in
+// complex64 the second set of registers costs more than it hides (groups of
+// 13 Paulis: 2.128 -> 2.686 ms), so those take one tile at a time.
+// NCSV_EXPECT_PREFETCH=0/1 forces either.
+bool expect_prefetch(bool dbl) {
+    const char *e = getenv("NCSV_EXPECT_PREFETCH");
+    return e ? atoi(e) != 0 : dbl;
+}
+
 std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo, bool dbl) {
     Gen g;
     // each exchange's LDS addresses are computed where it runs, not held across the tile loop
@@ -1452,40 +1463,76 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
     o << "__device__ __forceinline__ unsigned swz(unsigned t) { return t ^ (((t >> " << S << ") ^ (t >> " << 2 * S
       << ")) & " << ((1u << S) - 1) << "u); }\n";
     o << "__device__ __forceinline__ double wsum(double v) { for (int k = 32; k > 0; k >>= 1) v += __shfl_down(v, k, 64); return v; }\n";
+    // a barrier for the LDS exchanges that does not wait for the next tile's
+    // loads in flight (__syncthreads waits for every outstanding memory op)
+    const bool pf = expect_prefetch(dbl);
+    if (pf) {
+        o << "__device__ __forceinline__ void lds_bar() { asm volatile(\"s_waitcnt lgkmcnt(0)\\n\\ts_barrier\" ::: \"memory\"); }\n";
+        g.bar = "lds_bar()";
+    }
     o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT)
       << ") void ncsv_expect(const V *__restrict__ amp, double *__restrict__ part, unsigned long long ntiles) {\n";
     if (lds) o << "  __shared__ V lds[" << (1u << (g.half ? g.L - 1 : g.L)) << "];\n";
     o << "  __shared__ double red[" << ((1u << g.LT) / 64) << "][" << 2 * cnt << "];\n";
     o << "  const unsigned tid = threadIdx.x;\n";
     for (uint32_t k = 0; k < cnt; k++) o << "  double sr" << k << " = 0.0, si" << k << " = 0.0;\n";
-    o << "  for (unsigned long long bi = blockIdx.x; bi < ntiles; bi += gridDim.x) {\n";
-    // XCD-chunked order (the grid is a multiple of 8 workgroups, which go to
-    // the 8 XCDs in turn): XCD x walks the contiguous run of tiles x * ntiles/8 ...
-    if (nb >= 3 && default_border() != 0)
-        o << "  const unsigned long long b = ((bi & 7ull) << " << nb - 3 << ") | (bi >> 3);\n";
-    else
-        o << "  const unsigned long long b = bi;\n";
-    o << "  const unsigned long long base = " << deposit("b", g.oq, true) << ";\n";
     std::vector<uint32_t> low;
     for (uint32_t p : positions(memr, g.L, false)) low.push_back(eg.tq[p]);
     const std::vector<uint32_t> rpl = positions(memr, g.L, true);
-    {
-        std::ostringstream e;
-        e << "base";
-        for (uint32_t i = 0; i < g.LT; i++) e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low[i] << ")";
-        o << "  const unsigned long long gm = " << e.str() << ";\n";
-    }
     auto roff = [&](uint32_t r) {
         uint64_t off = 0;
         for (uint32_t j = 0; j < g.R; j++)
             if ((r >> j) & 1) off |= 1ull << eg.tq[rpl[j]];
         return off;
     };
-    for (uint32_t r = 0; r < g.NR; r++) {
-        if (jit_nt() & 2)
-            o << "  V a" << r << " = ld_nt(&amp[gm | " << roff(r) << "ull]);\n";
+    // tile index -> block index b (XCD-chunked order: the grid is a multiple
+    // of 8 workgroups, which go to the 8 XCDs in turn, so XCD x walks the
+    // contiguous run of tiles x * ntiles/8 ...) -> base -> loads into `dst`
+    auto emit_tile = [&](const std::string &ti, const std::string &sfx, const std::string &dst, const char *pre) {
+        if (nb >= 3 && default_border() != 0)
+            o << pre << "const unsigned long long b" << sfx << " = ((" << ti << " & 7ull) << " << nb - 3 << ") | (" << ti
+              << " >> 3);\n";
         else
-            o << "  V a" << r << " = amp[gm | " << roff(r) << "ull];\n";
+            o << pre << "const unsigned long long b" << sfx << " = " << ti << ";\n";
+        o << pre << "const unsigned long long base" << sfx << " = " << deposit("b" + sfx, g.oq, true) << ";\n";
+        if (dst.empty()) return;
+        std::ostringstream e;
+        e << "base" << sfx;
+        for (uint32_t i = 0; i < g.LT; i++) e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low[i] << ")";
+        o << pre << "const unsigned long long gm" << sfx << " = " << e.str() << ";\n";
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if (jit_nt() & 2)
+                o << pre << dst << r << " = ld_nt(&amp[gm" << sfx << " | " << roff(r) << "ull]);\n";
+            else
+                o << pre << dst << r << " = amp[gm" << sfx << " | " << roff(r) << "ull];\n";
+        }
+    };
+    if (pf) {
+        // software pipelined: the next tile's loads go out before this tile's
+        // exchanges and products, into a second set of registers
+        o << "  V";
+        for (uint32_t r = 0; r < g.NR; r++) o << (r ? ", a" : " a") << r;
+        o << ";\n  {\n";
+        emit_tile("(unsigned long long)blockIdx.x", "0", "a", "    ");
+        o << "  }\n";
+        o << "  for (unsigned long long bi = blockIdx.x; bi < ntiles; bi += gridDim.x) {\n";
+        emit_tile("bi", "", "", "  ");
+        o << "  V";
+        for (uint32_t r = 0; r < g.NR; r++) o << (r ? ", p" : " p") << r;
+        // unconditional (the last tile of a workgroup loads itself again, 1/128
+        // of the reads at 30 qubits): a conditional load leaves the compiler
+        // waiting for every load, the prefetch included, where the paths join
+        o << ";\n  {\n    const unsigned long long bj = bi + gridDim.x < ntiles ? bi + gridDim.x : bi;\n";
+        emit_tile("bj", "1", "p", "    ");
+        o << "  }\n";
+    } else {
+        o << "  for (unsigned long long bi = blockIdx.x; bi < ntiles; bi += gridDim.x) {\n";
+        emit_tile("bi", "", "", "  ");
+        o << "  V";
+        for (uint32_t r = 0; r < g.NR; r++) o << (r ? ", a" : " a") << r;
+        o << ";\n  {\n";
+        emit_tile("bi", "0", "a", "    ");
+        o << "  }\n";
     }
     uint32_t cur = memr;
     for (auto &st : eg.stages) {
@@ -1560,7 +1607,12 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
         }
         o << "  }\n";
     }
-    if (lds) o << "  __syncthreads();\n";
+    if (lds) o << "  " << g.bar << ";\n";
+    if (pf) {
+        // the prefetched tile becomes the current one (the loop body starts
+        // from logical register r in variable a<r>)
+        for (uint32_t r = 0; r < g.NR; r++) o << "  a" << r << " = p" << r << ";\n";
+    }
     o << "  }\n";
     o << "  const int lane = tid & 63, wv = tid >> 6;\n";
     for (uint32_t k = 0; k < cnt; k++) {
