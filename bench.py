@@ -8,9 +8,10 @@ amp-updates = (unfused gate count) x 2^n per step.  Prints one JSON line with
 the roofline of the dominant kernel (HIP-event timed on the state's stream),
 the precision form measured in the same run (QFT + 2000 RZ: 33 qubits
 complex64, 32 qubits complex128 and complex64, fidelity and max-abs error of
-complex64 against complex128), and a CPU baseline (the C oracle on a bounded
-sample of the 26-qubit member of the brickwork family, on all of this host's
-cores and on one thread).
+complex64 against complex128), the complex64 sweeps' FP32 MFMA blocks against
+the vector-only plan on a 30-qubit random SU(4) brickwork, and a CPU
+baseline (the C oracle on a bounded sample of the 26-qubit member of the
+brickwork family, on all of this host's cores and on one thread).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N]
 """
@@ -449,50 +450,109 @@ def run_frames(args):
                                  "detectors": f.num_detectors, "shots": shots}}), flush=True)
 
 
-def run_su4(args):
-    """A random two-qubit circuit: a line of Haar-random 4x4 unitaries on
-    alternating neighbour pairs, `--depth` layers on `--qubits` qubits,
-    complex64 by default; one step = set_basis(0) + the circuit.  Reports
-    the dominant kernel's roofline.  The sweeps' FP32 MFMA blocks follow
-    NCSV_JIT_MFMA (0, the default: none; 1: by cost)."""
+def run_stream(args):
+    """Streaming ceilings of the library's own elementwise kernels on an
+    n-qubit state (default 30, complex128): the FP64 norm reduction
+    (read-only), ncsv_scale (in-place read and write) and the plus-state fill
+    (write-only), timed by HIP events on the state's stream."""
     import ncsim_amd as H
-    from ncsim_amd import workloads as W
-    n, depth = args.qubits, args.depth
-    prec = H.C128 if args.precision == "c128" else H.C64
-    gates = W.su4_brickwork(n, depth, 3)
-    circ = W.build_circuit(n, gates, H.Circuit, H.Gate)
-    s = H.HipStateVector.zero_state(n, prec)
-    s.set_fusion(args.fusion)
-    for _ in range(max(1, args.warmup)):
-        s.set_basis(0)
-        s.run(circ)
+    prec = H.C64 if args.precision == "c64" else H.C128
+    s = H.HipStateVector.plus_state(args.qubits, prec)
     s.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        s.set_basis(0)
-        s.run(circ)
-    s.synchronize()
-    ms = (time.perf_counter() - t0) * 1000 / args.steps
-    s.set_basis(0)
+    s.norm_sq()
+    s.scale(1.0)
     s.profile_reset()
     s.profile(True)
-    s.run(circ)
+    for _ in range(args.steps):
+        s.norm_sq()
+        s.scale(1.0)
+        H._lib.lib().ncsv_set_plus(s._h)
     s.profile(False)
     st = s.profile_read()
+    out = {}
+    for k, (launches, ms, byts) in sorted(st.items()):
+        out[k] = {"launches": launches, "avg_ms": ms / launches, "bytes": byts / launches,
+                  "tb_per_s": byts / (ms / 1000.0) / 1e12, "frac_of_8tbs": byts / (ms / 1000.0) / HBM_PEAK}
+    print(json.dumps({"workload": f"streaming kernels on a {args.qubits}-qubit "
+                                  f"{'complex64' if prec == H.C64 else 'complex128'} state",
+                      "kernels": out}), flush=True)
+
+
+def su4_measure(n, depth, prec, mfma, steps=3, warmup=1, fusion=1):
+    """The random two-qubit circuit (a line of Haar-random 4x4 unitaries on
+    alternating neighbour pairs, `depth` layers) on an n-qubit state: one step
+    = set_basis(0) + the circuit, with the sweeps' FP32 MFMA blocks off
+    (mfma=0) or chosen by cost (mfma=1).  Returns ms per step, amp-updates/s
+    and the dominant kernel's roofline."""
+    import ncsim_amd as H
+    from ncsim_amd import workloads as W
+    old = os.environ.get("NCSV_JIT_MFMA")
+    os.environ["NCSV_JIT_MFMA"] = str(mfma)
+    try:
+        gates = W.su4_brickwork(n, depth, 3)
+        circ = W.build_circuit(n, gates, H.Circuit, H.Gate)
+        s = H.HipStateVector.zero_state(n, prec)
+        s.set_fusion(fusion)
+        for _ in range(max(1, warmup)):
+            s.set_basis(0)
+            s.run(circ)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.set_basis(0)
+            s.run(circ)
+        s.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / steps
+        s.set_basis(0)
+        s.profile_reset()
+        s.profile(True)
+        s.run(circ)
+        s.profile(False)
+        st = s.profile_read()
+        sweeps = s.sweep_count()
+        del s
+    finally:
+        if old is None:
+            os.environ.pop("NCSV_JIT_MFMA", None)
+        else:
+            os.environ["NCSV_JIT_MFMA"] = old
     name, (launches, tot_ms, byts) = max(st.items(), key=lambda kv: kv[1][1])
     achieved = byts / launches / (tot_ms / launches / 1000.0)
-    log("su4: kernels " + ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(st.items())))
-    print(json.dumps({"metric": "amp-updates/s", "value": len(gates) * float(1 << n) / (ms / 1000.0),
-                      "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": max(1, args.warmup),
-                      "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+    log(f"su4: n={n} depth={depth} mfma={mfma} {ms:.1f} ms/step  " +
+        ", ".join(f"{k}: {v[0]} x {v[1] / v[0]:.3f} ms" for k, v in sorted(st.items())))
+    return {"ms_per_step": ms, "value": len(gates) * float(1 << n) / (ms / 1000.0), "gates": len(gates),
+            "sweeps_per_step": sweeps,
+            "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK, "launches": launches,
+                         "avg_ms": tot_ms / launches}}
+
+
+def mfma_blocks(n=30, depth=12, steps=2):
+    """FP32 MFMA dense blocks against the vector-only sweeps on the random
+    SU(4) brickwork, complex64, measured in the same run."""
+    import ncsim_amd as H
+    on = su4_measure(n, depth, H.C64, 1, steps)
+    off = su4_measure(n, depth, H.C64, 0, steps)
+    return {"workload": f"random SU(4) brickwork {n} qubits depth {depth}, complex64", "gates": on["gates"],
+            "mfma_by_cost": on, "vector_only": off, "mfma_over_vector_time": on["ms_per_step"] / off["ms_per_step"],
+            "default": "vector only (NCSV_JIT_MFMA=0)"}
+
+
+def run_su4(args):
+    """The random SU(4) brickwork alone (complex64 unless --precision c128);
+    MFMA blocks as NCSV_JIT_MFMA says (default 0: none)."""
+    import ncsim_amd as H
+    prec = H.C128 if args.precision == "c128" else H.C64
+    r = su4_measure(args.qubits, args.depth, prec, int(os.environ.get("NCSV_JIT_MFMA", "0")), args.steps,
+                    args.warmup, args.fusion)
+    print(json.dumps({"metric": "amp-updates/s", "value": r["value"], "unit": "amp-updates/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": r["ms_per_step"],
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                       "dtype": "c64 (f32, f64 reductions)" if prec == H.C64 else "c128 (f64)",
                       "data": "synthetic (seeded circuit)",
-                      "config": {"workload": f"random SU(4) brickwork {n} qubits depth {depth}", "gates": len(gates),
-                                 "mfma_blocks": os.environ.get("NCSV_JIT_MFMA", "0")},
-                      "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
-                                   "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                                   "launches": launches, "avg_ms": tot_ms / launches,
-                                   "sweeps_per_step": s.sweep_count()}}), flush=True)
+                      "config": {"workload": f"random SU(4) brickwork {args.qubits} qubits depth {args.depth}",
+                                 "gates": r["gates"], "mfma_blocks": os.environ.get("NCSV_JIT_MFMA", "0")},
+                      "roofline": r["roofline"], "sweeps_per_step": r["sweeps_per_step"]}), flush=True)
 
 
 def main():
@@ -507,7 +567,7 @@ def main():
     ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--only-n", action="store_true", help="qft workload: only the n-qubit complex64 runs")
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "qft", "batch", "trajectories", "frames", "copy", "su4"])
+                    choices=["brickwork", "qft", "batch", "trajectories", "frames", "copy", "su4", "stream"])
     ap.add_argument("--distance", type=int, default=25, help="frames workload: repetition-code distance")
     ap.add_argument("--rounds", type=int, default=25, help="frames workload: rounds")
     ap.add_argument("--shots", type=int, default=1 << 24, help="frames workload: shots per step")
@@ -542,6 +602,9 @@ def main():
         return
     if args.workload == "su4":
         run_su4(args)
+        return
+    if args.workload == "stream":
+        run_stream(args)
         return
 
     import ncsim_amd as H
@@ -595,9 +658,10 @@ def main():
                 "algorithmic_bytes": byts / launches, "launches": launches,
                 "avg_ms": tot_ms / launches, "sweeps_per_step": sweeps}
     del s
-    pf = None
+    pf = mb = None
     if not args.no_precision_form and n == 30:
         pf = precision_form(33, max(1, min(args.steps, 3)), args.fusion)
+        mb = mfma_blocks()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "amp-updates/s", "value": value, "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -606,7 +670,7 @@ def main():
         "config": {"workload": f"rotation brickwork {n} qubits depth {depth} + 64 Pauli expectations",
                    "qubits": n, "depth": depth, "gates": gates, "fusion": args.fusion,
                    "precision": "complex128" if prec == H.C128 else "complex64"},
-        "roofline": roofline, "precision_form": pf, "cpu_baseline": cpu,
+        "roofline": roofline, "precision_form": pf, "mfma_blocks": mb, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

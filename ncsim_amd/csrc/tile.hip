@@ -540,6 +540,7 @@ TileGeom tile_geom(ncsv_precision prec) {
     if (const char *e = getenv("NCSV_TILE_R")) g.R = (uint32_t)atoi(e);
     if (const char *e = getenv("NCSV_TILE_C")) g.c = (uint32_t)atoi(e);
     if (const char *e = getenv("NCSV_TILE_PIPE")) g.pipe = atoi(e) != 0;
+    if (const char *e = getenv("NCSV_FLOOR_OPS")) g.floor_ops = (uint32_t)atoi(e);
     return g;
 }
 
