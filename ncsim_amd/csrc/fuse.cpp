@@ -497,8 +497,33 @@ SweepH build_sweep(const std::vector<Cls> &cl, const std::vector<uint32_t> &chos
     return sw;
 }
 
+// Vector instructions per amplitude a gate costs in a register sweep (the
+// planner's balance estimate): 2x2s with real entries or a real diagonal
+// and imaginary off-diagonal ~2.5, other 2x2s ~8.5, 4x4s ~16.5, phases and
+// flips (applied lazily, or renamed) ~0.5.
+double op_weight(const Cls &c) {
+    switch (c.kind) {
+    case TK_U1: {
+        auto re = [](cd z) { return z.imag() == 0.0; };
+        auto im = [](cd z) { return z.real() == 0.0; };
+        const bool real = re(c.m[0]) && re(c.m[1]) && re(c.m[2]) && re(c.m[3]);
+        const bool xform = re(c.m[0]) && re(c.m[3]) && im(c.m[1]) && im(c.m[2]);
+        return (real || xform) ? 2.5 : 8.5;
+    }
+    case TK_U2: return 16.5;
+    case TK_SWAPR: return 0.0;
+    default: return 0.5;
+    }
+}
+
+// cap > 0: a sweep admits its gates in program order only while their
+// summed weight stays within cap (at least one gate); the rest wait for a
+// later sweep (a prefix of the admissible gates is closed under their
+// dependencies).
 std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t> rem, uint32_t n,
-                                const TileGeom &geo, uint64_t first_fixed = 0, uint32_t variant = 0) {
+                                const TileGeom &geo, uint64_t first_fixed = 0, uint32_t variant = 0,
+                                double cap = 0.0, std::vector<std::vector<uint32_t>> *gots = nullptr,
+                                std::vector<uint64_t> *sets = nullptr) {
     std::vector<SweepH> out;
     const uint32_t L = geo.L();
     const uint64_t fixed = (1ull << std::min(geo.c, n)) - 1;
@@ -508,7 +533,15 @@ std::vector<SweepH> plan_sweeps(const std::vector<Cls> &cl, std::vector<uint32_t
         std::vector<uint32_t> got;
         pick(cl, rem, T, &got, nullptr);
         if (got.empty()) fail(NCSV_INVALID, "sweep planner made no progress");
+        if (cap > 0.0) {
+            double w = 0.0;
+            size_t keep = 0;
+            while (keep < got.size() && (keep == 0 || w + op_weight(cl[got[keep]]) <= cap)) w += op_weight(cl[got[keep++]]);
+            got.resize(keep);
+        }
         out.push_back(build_sweep(cl, got, T, n, geo));
+        if (gots) gots->push_back(got);
+        if (sets) sets->push_back(T);
         std::vector<uint32_t> left;
         size_t gi = 0;
         for (uint32_t i : rem) {
@@ -681,6 +714,64 @@ bool use_specialized(const ncsv_state *s) {
     return (int)s->n >= minq;
 }
 
+// Whether the planner evens out the arithmetic of a plan's sweeps
+// (NCSV_PLAN_BALANCE=0: never).
+bool balance_plans() {
+    const char *e = getenv("NCSV_PLAN_BALANCE");
+    return e ? atoi(e) != 0 : true;
+}
+
+// Evens out a plan without changing its sweeps' tiles or count: a sweep
+// whose gates cost more arithmetic than a pass over memory hides (W0 vector
+// instructions per amplitude) hands gates to the next sweep, latest first,
+// while the pair's larger cost goes down.  A gate may move when it commutes
+// with every later gate of its sweep (so the gates left stay closed under
+// their dependencies) and the next sweep's tile holds the qubits it acts on
+// non-diagonally; it commutes with the next sweep's earlier gates by
+// construction (those were left out of its sweep, and it was not).  Returns
+// whether anything moved.
+bool balance_sweeps(const std::vector<Cls> &cl, uint32_t n, const TileGeom &geo,
+                    std::vector<std::vector<uint32_t>> *gots, const std::vector<uint64_t> &sets) {
+    (void)n;
+    const double W0 = geo.floor_ops >= 100 ? 128.0 : 64.0;
+    auto weight = [&](const std::vector<uint32_t> &g) {
+        double w = 0.0;
+        for (uint32_t i : g) w += op_weight(cl[i]);
+        return w;
+    };
+    auto commute = [&](const Cls &a, const Cls &b) { return !(a.act & b.sup) && !(b.act & a.sup); };
+    bool any = false;
+    for (int pass = 0; pass < 8; pass++) {
+        bool moved = false;
+        for (size_t k = 0; k + 1 < gots->size(); k++) {
+            std::vector<uint32_t> &A = (*gots)[k], &B = (*gots)[k + 1];
+            double wa = weight(A), wb = weight(B);
+            while (wa > W0) {
+                // the latest movable gate of A
+                int pick = -1;
+                for (int j = (int)A.size() - 1; j >= 0 && pick < 0; j--) {
+                    const Cls &g = cl[A[(size_t)j]];
+                    if (g.act & ~sets[k + 1]) continue;
+                    const double w = op_weight(g);
+                    if (w <= 0.0 || wb + w >= wa) continue;
+                    bool ok = true;
+                    for (size_t t = (size_t)j + 1; t < A.size() && ok; t++) ok = commute(g, cl[A[t]]);
+                    if (ok) pick = j;
+                }
+                if (pick < 0) break;
+                const uint32_t gi = A[(size_t)pick];
+                A.erase(A.begin() + pick);
+                B.insert(std::lower_bound(B.begin(), B.end(), gi), gi);
+                wa -= op_weight(cl[gi]);
+                wb += op_weight(cl[gi]);
+                moved = any = true;
+            }
+        }
+        if (!moved) break;
+    }
+    return any;
+}
+
 std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const TileGeom &geo) {
     std::vector<SegItem> items;
     std::vector<Cls> cl;
@@ -703,6 +794,7 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
         };
         if (n >= 20) {
             auto bs = score(best);
+            uint32_t bv = 0;
             for (uint32_t v = 1; v < 24; v++) {
                 std::vector<SweepH> c;
                 try {
@@ -714,7 +806,15 @@ std::vector<SegItem> plan_segment(const std::vector<LOp> &ops, uint32_t n, const
                 if (cs < bs) {
                     best.swap(c);
                     bs = cs;
+                    bv = v;
                 }
+            }
+            if (balance_plans()) {
+                std::vector<std::vector<uint32_t>> gots;
+                std::vector<uint64_t> sets;
+                (void)plan_sweeps(cl, run, n, geo, 0, bv, 0.0, &gots, &sets);
+                if (balance_sweeps(cl, n, geo, &gots, sets))
+                    for (size_t k = 0; k < gots.size(); k++) best[k] = build_sweep(cl, gots[k], sets[k], n, geo);
             }
         }
         for (auto &sw : best) {
