@@ -199,8 +199,8 @@ def test_generated_sweeps_on_host_threads(name, precision, make, half, monkeypat
 # state.  The block order is a compile-time constant, so each case runs in a
 # fresh process.
 SHAPES = [
-    ("outer-load2", H.C128, lambda: outer_heavy(21, 12, 4), {"NCSV_JIT_MEMSHAPE": "2"}),
-    ("outer-load3-store5-border", H.C128, lambda: outer_heavy(21, 12, 4),
+    ("outer-load2", H.C128, lambda: outer_heavy(16, 12, 4), {"NCSV_JIT_MEMSHAPE": "2"}),
+    ("outer-load3-store5-border", H.C128, lambda: outer_heavy(16, 12, 4),
      {"NCSV_JIT_MEMSHAPE": "3", "NCSV_JIT_MEMSHAPE_STORE": "5", "NCSV_JIT_BORDER": "1"}),
     ("brickwork-c64-first-last", H.C64, lambda: W.rotation_brickwork(17, 4, 3),
      {"NCSV_JIT_MEMSHAPE": "5", "NCSV_JIT_MEMSHAPE_STORE": "6", "NCSV_JIT_BORDER": "1"}),
@@ -251,11 +251,11 @@ def su4_circuit(n, depth, seed):
 
 
 MFMA_CASES = [
-    ("su4-auto", lambda: su4_circuit(15, 6, 1), {"NCSV_JIT_MFMA": "1"}),
-    ("su4-multi-tile", lambda: su4_circuit(17, 5, 2), {"NCSV_JIT_MFMA": "2"}),
-    ("brickwork-forced", lambda: W.rotation_brickwork(15, 4, 3), {"NCSV_JIT_MFMA": "2"}),
-    ("qft-forced", lambda: W.qft_with_rz(15, 12, 2), {"NCSV_JIT_MFMA": "2"}),
-    ("outer-forced", lambda: outer_heavy(15, 13, 5), {"NCSV_JIT_MFMA": "2"}),
+    ("su4-auto", lambda: su4_circuit(14, 6, 1), {"NCSV_JIT_MFMA": "1"}),
+    ("su4-multi-tile", lambda: su4_circuit(16, 4, 2), {"NCSV_JIT_MFMA": "2"}),
+    ("brickwork-forced", lambda: W.rotation_brickwork(14, 3, 3), {"NCSV_JIT_MFMA": "2"}),
+    ("qft-forced", lambda: W.qft_with_rz(14, 8, 2), {"NCSV_JIT_MFMA": "2"}),
+    ("outer-forced", lambda: outer_heavy(14, 13, 5), {"NCSV_JIT_MFMA": "2"}),
 ]
 
 
