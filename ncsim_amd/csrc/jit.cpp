@@ -1306,6 +1306,27 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
         return off;
     };
     auto roff = [&](uint32_t r) { return roff_of(rpl, r); };
+    if (from_basis) {
+        // a tile without the basis index starts at zero and stays zero (the
+        // sweep's non-diagonal ops act on tile qubits only; its phases map 0
+        // to 0): its workgroup stores zeros and returns (a workgroup-uniform
+        // branch), so only the basis index's tile does the arithmetic
+        uint64_t outer = 0;
+        for (uint32_t q : g.oq) outer |= 1ull << q;
+        o << "  if (base != (binit & " << outer << "ull)) {\n";
+        std::ostringstream e;
+        e << "base";
+        for (uint32_t i = 0; i < g.LT; i++)
+            e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low_s[i] << ")";
+        o << "    const unsigned long long gz = " << e.str() << ";\n    V z; z.x = (T)0; z.y = (T)0;\n";
+        for (uint32_t r = 0; r < g.NR; r++) {
+            if (nt & 1)
+                o << "    st_nt(&amp[gz | " << roff_of(rps, r) << "ull], z);\n";
+            else
+                o << "    amp[gz | " << roff_of(rps, r) << "ull] = z;\n";
+        }
+        o << "    return;\n  }\n";
+    }
     for (uint32_t r = 0; r < g.NR; r++) {
         if (from_basis)
             o << "  V a" << r << "; a" << r << ".x = (T)((gm | " << roff(r) << "ull) == binit ? 1 : 0); a" << r
@@ -2001,6 +2022,12 @@ ncsv_status ncsv_jit_check(const ncsv_circuit *c, ncsv_precision prec, const cha
             const std::string src = jit::generate(it.sw, c->n, geo, prec == NCSV_C128, &coef, false, &ms);
             std::string log;
             if (jit::compile_to_code(src, arch ? arch : "gfx950", &log).empty()) fail(NCSV_DEVICE, "compile: " + log);
+            if (k == 0) {
+                // the first sweep's form for runs from a basis state
+                std::vector<double> cb;
+                const std::string sb = jit::generate(it.sw, c->n, geo, prec == NCSV_C128, &cb, true, &ms);
+                if (jit::compile_to_code(sb, arch ? arch : "gfx950", &log).empty()) fail(NCSV_DEVICE, "compile: " + log);
+            }
             k++;
         }
         if (nsweeps) *nsweeps = k;

@@ -98,7 +98,7 @@ extern "C" void run_host(V *amp, unsigned long long nblocks, unsigned nthreads, 
 HOST_NT = "inline V ld_nt(const V *p) { return *p; }\ninline void st_nt(V *p, V v) { *p = v; }"
 
 
-def host_kernel(src, workdir, name):
+def host_kernel(src, workdir, name, post=None):
     lines = []
     for line in src.splitlines():
         if ("asm volatile" in line or "ext_vector_type" in line or "__builtin_nontemporal" in line
@@ -111,7 +111,7 @@ def host_kernel(src, workdir, name):
     src = src.replace('extern "C" ', "")
     cpp = os.path.join(workdir, name + ".cpp")
     with open(cpp, "w") as f:
-        f.write(PRE + src + POST)
+        f.write(PRE + src + (POST if post is None else post))
     so = os.path.join(workdir, name + ".so")
     r = subprocess.run(["g++", "-std=c++20", "-O1", "-shared", "-fPIC", "-o", so, cpp, "-lpthread"],
                        capture_output=True, text=True)
@@ -323,3 +323,65 @@ def test_mfma_blocks_on_host_threads(name):
     p.join(60)
     assert not isinstance(r, str), r
     assert r < 2e-6, (name, r)
+
+
+POST_BASIS = r'''
+extern "C" void run_host_basis(V *amp, unsigned long long nblocks, unsigned nthreads, const T *coef,
+                               unsigned long long binit) {
+    for (unsigned long long b = 0; b < nblocks; b++) {
+        std::barrier<> bar(nthreads);
+        g_bar = &bar;
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nthreads; t++)
+            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_sweep_from_basis(amp, coef, binit); });
+        for (auto &x : th) x.join();
+    }
+}
+'''
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ with C++20")
+@pytest.mark.parametrize("precision,index", [(H.C128, 0), (H.C128, 40961), (H.C64, 70001)])
+def test_from_basis_sweep_on_host_threads(precision, index, monkeypatch):
+    """A plan's first sweep in its from-basis form (the basis amplitude made
+    in registers; tiles without the basis index store zeros and return)
+    followed by the plan's other sweeps, on host threads, equals the plan's
+    FP64 emulation from the same basis state (17 qubits: 32 or 16 tiles)."""
+    n = 17
+    text = W.rotation_brickwork(n, 4, 9)
+    work = tempfile.mkdtemp()
+    try:
+        monkeypatch.setenv("NCSV_JIT_DUMP", work)
+        c = H.Circuit.parse(text)
+        k = ctypes.c_uint32()
+        assert L.lib().ncsv_jit_check(c._h, int(precision), b"gfx950", ctypes.byref(k)) == 0
+        order = [line.split() for line in open(os.path.join(work, "order.txt")).read().splitlines()]
+        hashes = [h for h, fb in order if fb == "0"]
+        want = np.zeros(1 << n, dtype=np.complex128)
+        want[index] = 1.0
+        stats = (ctypes.c_uint64 * 4)()
+        assert L.lib().ncsv_plan_emulate(c._h, int(precision), want.view(np.float64).ctypes.data_as(L.c_dp),
+                                         stats) == 0
+        buf = np.full(1 << n, np.nan + 1j * np.nan, dtype=np.complex128 if precision == H.C128 else np.complex64)
+        tile = 12 if precision == H.C128 else 13
+        for i, h in enumerate(hashes):
+            src = open(os.path.join(work, f"ncsv_{h}.hip")).read()
+            coef = np.fromfile(os.path.join(work, f"ncsv_{h}.coef"), dtype=np.float64)
+            coef = np.ascontiguousarray(coef if precision == H.C128 else coef.astype(np.float32))
+            if i == 0:
+                fb = [f for f, flag in order if flag == "1"]
+                assert fb, "the check wrote no from-basis form"
+                src_b = open(os.path.join(work, f"ncsv_{fb[0]}.hip")).read()
+                lib2 = host_kernel(src_b, work, "kb", POST_BASIS)
+                lib2.run_host_basis(buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_ulonglong(1 << (n - tile)),
+                                    ctypes.c_uint(256), coef.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_ulonglong(index))
+                continue
+            lib = host_kernel(src, work, f"k{i}")
+            lib.run_host(buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_ulonglong(1 << (n - tile)), ctypes.c_uint(256),
+                         coef.ctypes.data_as(ctypes.c_void_p))
+        got = buf.astype(np.complex128)
+        assert not np.isnan(got).any()
+        assert np.abs(got - want).max() < (1e-13 if precision == H.C128 else 2e-6)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
