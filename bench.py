@@ -205,6 +205,8 @@ def run_batch(args):
 
     prec = H.C128 if args.precision == "c128" else H.C64
     specs = W.batch_circuits(args.circuits, 11)
+    if args.qmin or args.qmax:  # one size band of the mix
+        specs = [sp for sp in specs if (args.qmin or 0) <= sp[0] <= (args.qmax or 64)]
     circs = [H.Circuit.parse(t) for _, t, _ in specs]
     paulis = [[H.Pauli(n, x, z, ph) for x, z, ph in ps] for n, _, ps in specs]
     gates = np.array([W.count_gates(t) for _, t, _ in specs], dtype=np.float64)
@@ -284,16 +286,22 @@ def run_batch(args):
     per, maxdiff, d128 = per_circuit(allidx[::args.stride], b, {i: i for i in allidx})
     del b
     small = [i for i in allidx if ns[i] <= 14]
-    bs, wall_s, kms_s, _ = batched(small)
-    per_s, maxdiff_s, d128_s = per_circuit(small[::max(1, args.stride // 2)], bs, {i: k for k, i in enumerate(small)})
-    del bs
+    if small:
+        bs, wall_s, kms_s, _ = batched(small)
+        per_s, maxdiff_s, d128_s = per_circuit(small[::max(1, args.stride // 2)], bs,
+                                               {i: k for k, i in enumerate(small)})
+        del bs
+    else:
+        wall_s = kms_s = per_s = 1.0
+        maxdiff_s, d128_s = 0.0, (None if d128 is None else 0.0)
     amp_updates = float((gates * np.exp2(ns)).sum())
     cps = len(specs) / wall
     out = {"metric": "circuits/s", "value": cps, "unit": "circuits/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall * 1000, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "c64 (f32, f64 reductions)" if prec == H.C64 else "c128 (f64)",
            "data": "synthetic (seeded circuits)",
-           "config": {"workload": f"{len(specs)} circuits, 10-22 qubits, depth 10-120, 8 Paulis each",
+           "config": {"workload": f"{len(specs)} circuits, {int(ns.min())}-{int(ns.max())} qubits, depth 10-120, "
+                                  f"8 Paulis each",
                       "gates": int(gates.sum()), "tile_bits": args.tile_bits or None},
            "amp_updates_per_s": amp_updates / wall, "kernel_ms": kms, "first_run_s": cold,
            "per_circuit": {"circuits_per_s": 1.0 / per, "sample": len(allidx[::args.stride]),
@@ -578,6 +586,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["c128", "c64"],
                     help="default: complex128 for the brickwork, complex64 for the batch")
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--qmin", type=int, default=0, help="batch workload: only circuits of at least this many qubits")
+    ap.add_argument("--qmax", type=int, default=0, help="batch workload: only circuits of at most this many qubits")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
