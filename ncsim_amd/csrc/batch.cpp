@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <future>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 
@@ -26,6 +28,7 @@ namespace batch {
 uint32_t max_tile_bits(ncsv_precision prec);
 int resident_blocks(ncsv_precision prec, uint32_t threads);
 void launch(const BArgs &A, uint32_t nrec, ncsv_precision prec, uint32_t threads, int blocks, hipStream_t stream);
+void launch_finish(const double *part, const BSumD *sums, uint32_t count, double *out, hipStream_t stream);
 void simulate(const BArgs &A, ncsv_precision prec, uint32_t threads);
 
 struct StepH {
@@ -334,6 +337,13 @@ bool block_fusion_enabled() {
     return !(e && e[0] == '0');
 }
 
+// Complex128 dense blocks on two positions run as 4x4s on the VALU unless
+// NCSV_BATCH_U2=0 (then as 8x8s with an identity factor, on the MFMA).
+bool u2_blocks() {
+    const char *e = getenv("NCSV_BATCH_U2");
+    return !(e && e[0] == '0');
+}
+
 // Ops fused into dense 8x8 blocks on three tile positions (BK_U3: FP32 MFMA
 // in complex64 programs, FP64 MFMA in complex128 ones).  Each position belongs to at most one open
 // block; an op whose positions fit one block (together with the blocks it
@@ -341,8 +351,8 @@ bool block_fusion_enabled() {
 // block is emitted at its close with every op it commutes past acting on other
 // positions.  A closed block becomes one BK_U3 when it has two or more
 // positions, three or more ops and one op on two or more positions (a run of
-// one-position ops is cheaper as grouped 2x2s); otherwise its ops stay as
-// they were.
+// one-position ops is cheaper as grouped 2x2s; in complex128 a block on two
+// positions becomes one BK_U2 4x4); otherwise its ops stay as they were.
 std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in, bool fp32_copy) {
     struct Blk {
         uint32_t mask;
@@ -411,6 +421,27 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in, bool fp32_c
         }
         uint32_t pos[3], k = 0;
         for (uint32_t m = b.mask; m; m &= m - 1) pos[k++] = (uint32_t)__builtin_ctz(m);
+        if (k == 2 && !b.omask && !fp32_copy && u2_blocks()) {
+            // complex128: a block on two positions as one 4x4 on the VALU, half
+            // the arithmetic of the 8x8 with an identity factor (FP64 MFMA runs
+            // at the FP64 VALU rate on this part; 21-22-qubit circuits of the
+            // mix: 674.5 -> 664.2 ms)
+            cd M[16];
+            for (uint32_t c = 0; c < 4; c++) {
+                cd v[4] = {};
+                v[c] = 1;
+                for (const BOpD &o : b.ops) apply_local(*s, o, pos, 2, v, false);
+                for (uint32_t r = 0; r < 4; r++) M[4 * r + c] = v[r];
+            }
+            BOpD d{};
+            d.kind = BK_U2;
+            d.p0 = (uint8_t)pos[0];
+            d.p1 = (uint8_t)pos[1];
+            d.coef = (uint32_t)s->coef.size();
+            s->coef.insert(s->coef.end(), M, M + 16);
+            out.push_back(d);
+            return;
+        }
         for (uint32_t p = 0; k < 3; p++)
             if (!((b.mask >> p) & 1)) pos[k++] = p;  // identity on an unused position
         BOpD d{};
@@ -900,24 +931,37 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         }
     }
     if (getenv("NCSV_BATCH_DUMP")) {
+        // programs are planned on several threads: each one's report is
+        // assembled first and written in one piece
         static const char *nm[] = {"U1", "U1R", "U1X", "U2", "DM", "PERM1", "SWAP", "NSLOT",
                                    "MEAS", "RED", "COLL", "EXP", "G3", "U3", "OPH"};
+        std::string txt;
+        char buf[256];
         for (size_t i = 0; i < P->steps.size(); i++) {
             const StepH &st = P->steps[i];
             int cnt[15] = {0};
             for (const BOpD &o : st.gopsc) cnt[o.kind]++;
-            fprintf(stderr, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu one-error-lists=%zu:", i, st.L,
-                    st.flags, st.gops.size(), st.gopsc.size(), st.slots.size(), st.gops1.size());
+            snprintf(buf, sizeof buf, "step %zu L=%u flags=%u ops=%zu clean=%zu slots=%zu one-error-lists=%zu:", i,
+                     st.L, st.flags, st.gops.size(), st.gopsc.size(), st.slots.size(), st.gops1.size());
+            txt += buf;
             for (int k = 0; k < 15; k++)
-                if (cnt[k]) fprintf(stderr, " %s=%d", nm[k], cnt[k]);
-            fprintf(stderr, "\n");
+                if (cnt[k]) {
+                    snprintf(buf, sizeof buf, " %s=%d", nm[k], cnt[k]);
+                    txt += buf;
+                }
+            txt += "\n";
             if (atoi(getenv("NCSV_BATCH_DUMP")) >= 2) {
                 // the slot-free op list: kind, positions, in-tile and outer condition masks
-                for (const BOpD &o : st.gopsc)
-                    fprintf(stderr, "  %s p=%u,%u,%u l=%x o=%llx\n", nm[o.kind], o.p0, o.p1, o.p2, o.lmask,
-                            (unsigned long long)o.omask);
+                for (const BOpD &o : st.gopsc) {
+                    snprintf(buf, sizeof buf, "  %s p=%u,%u,%u l=%x o=%llx\n", nm[o.kind], o.p0, o.p1, o.p2, o.lmask,
+                             (unsigned long long)o.omask);
+                    txt += buf;
+                }
             }
         }
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        fputs(txt.c_str(), stderr);
     }
     for (uint32_t s = 0; s < P->steps.size(); s++) {
         P->items += P->tiles(s);
@@ -1218,7 +1262,29 @@ struct Chunk {
     std::vector<uint8_t> forced, errs, outc;
     bool any_forced = false;
     std::vector<double> pexp;
+    // every entry's expectation sums (Paulis, then the norm; real and
+    // imaginary parts), where their per-tile partials sit in pexp, and their
+    // values before the entry's scale
+    std::vector<BSumD> sums;
+    std::vector<double> esum;
 };
+
+// The sums of per-tile partials on the host (the device's k_batch_finish
+// sequence): in tile order, Neumaier-compensated.
+void sum_partials(Chunk *ch) {
+    ch->esum.resize(ch->sums.size());
+    for (size_t i = 0; i < ch->sums.size(); i++) {
+        const BSumD &r = ch->sums[i];
+        double s = 0.0, comp = 0.0;
+        for (uint32_t t = 0; t < r.tiles; t++) {
+            const double x = ch->pexp[r.off + (uint64_t)r.stride * t];
+            const double y = s + x;
+            comp += std::fabs(s) >= std::fabs(x) ? (s - y) + x : (x - y) + s;
+            s = y;
+        }
+        ch->esum[i] = s + comp;
+    }
+}
 
 // Draws of one entry: uniforms from the caller's array or its SplitMix64
 // seed, in program order; noise draws become per-slot error codes
@@ -1278,6 +1344,15 @@ Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *
         ch.nmeas += P.nmeas;
         d.err0 = ch.nslots;
         ch.nslots += P.nslots;
+    }
+    for (size_t i = 0; i < ne; i++) {
+        const auto &E = b->entries[ent[i]];
+        const Program &P = *b->progs[E.prog];
+        for (size_t k = 0; k <= E.x.size(); k++) {
+            const ExpRef &r = P.exps[k];
+            for (uint32_t c = 0; c < 2; c++)
+                ch.sums.push_back(BSumD{ch.ed[i].exp0 + r.off + 2ull * r.k + c, 2 * r.K, r.tiles});
+        }
     }
     ch.uni.assign(ch.nmeas + 1, 0.0);
     ch.forced.assign(ch.nmeas + 1, 0xff);
@@ -1364,7 +1439,8 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
                  o_blk = take(ch.rec_blk.size() * 4), o_part = take((ch.npart + 16) * 8),
                  o_exp = take((ch.nexp + 1) * 8), o_lsc = take(ch.lsc.size() * 8), o_uni = take(ch.uni.size() * 8),
                  o_forced = take(ch.forced.size()), o_outc = take(ch.nmeas + 1), o_errs = take(ch.errs.size()),
-                 o_done = take(ch.recs.size() * 4), o_next = take(8), o_flag = take(4);
+                 o_done = take(ch.recs.size() * 4), o_next = take(8), o_flag = take(4),
+                 o_sums = take(ch.sums.size() * sizeof(BSumD)), o_esum = take(ch.sums.size() * 8 + 8);
     ensure(&b->d_work, &b->work_cap, off, "hipMalloc(batch work)");
     char *w = (char *)b->d_work;
     hipStream_t st = b->stream;
@@ -1378,6 +1454,7 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
     up(o_uni, ch.uni.data(), ch.uni.size() * 8);
     up(o_forced, ch.forced.data(), ch.forced.size());
     up(o_errs, ch.errs.data(), ch.errs.size());
+    up(o_sums, ch.sums.data(), ch.sums.size() * sizeof(BSumD));
     BArgs A{};
     A.steps = at<const BStepD>(b->d_tab, b->off_steps);
     A.ops = at<const BOpD>(b->d_tab, b->off_ops);
@@ -1437,11 +1514,14 @@ void run_chunk_device(ncsv_batch *b, Chunk &ch) {
         check_hip(hipGraphLaunch(b->gexec, st), "hipGraphLaunch");
     else
         launch(A, (uint32_t)ch.recs.size(), b->prec, threads, blocks, st);
+    // the per-tile partials summed on the device: only the sums come back
+    launch_finish(A.pexp, (const BSumD *)(w + o_sums), (uint32_t)ch.sums.size(), (double *)(w + o_esum), st);
     check_hip(hipEventRecord(b->ev1, st), "hipEventRecord");
-    ch.pexp.assign(ch.nexp + 1, 0.0);
+    ch.esum.assign(ch.sums.size(), 0.0);
     ch.outc.assign(ch.nmeas + 1, 0xff);
     uint32_t flag = 0;
-    check_hip(hipMemcpyAsync(ch.pexp.data(), w + o_exp, ch.pexp.size() * 8, hipMemcpyDeviceToHost, st), "download");
+    check_hip(hipMemcpyAsync(ch.esum.data(), w + o_esum, ch.esum.size() * 8, hipMemcpyDeviceToHost, st),
+              "download");
     check_hip(hipMemcpyAsync(ch.lsc.data(), w + o_lsc, ch.lsc.size() * 8, hipMemcpyDeviceToHost, st), "download");
     check_hip(hipMemcpyAsync(ch.outc.data(), w + o_outc, ch.outc.size(), hipMemcpyDeviceToHost, st), "download");
     check_hip(hipMemcpyAsync(&flag, w + o_flag, 4, hipMemcpyDeviceToHost, st), "download");
@@ -1734,6 +1814,7 @@ void emulate_chunk(ncsv_batch *b, const Tables &t, Chunk &ch) {
 }
 
 void collect(ncsv_batch *b, const Chunk &ch) {
+    size_t si = 0;  // the entries' sums follow one another in ch.esum
     for (size_t i = 0; i < ch.ent.size(); i++) {
         const uint32_t e = ch.ent[i];
         const auto &E = b->entries[e];
@@ -1742,20 +1823,7 @@ void collect(ncsv_batch *b, const Chunk &ch) {
         const double l2 = ch.lsc[i] * ch.lsc[i];
         const size_t np = E.x.size();
         std::vector<double> v(2 * (np + 1));
-        for (size_t k = 0; k <= np; k++) {
-            const ExpRef &r = P.exps[k];
-            // compensated sum over tiles (Neumaier)
-            for (int c = 0; c < 2; c++) {
-                double s = 0.0, comp = 0.0;
-                for (uint32_t t = 0; t < r.tiles; t++) {
-                    const double x = ch.pexp[en.exp0 + r.off + 2ull * r.K * t + 2 * r.k + c];
-                    const double y = s + x;
-                    comp += std::fabs(s) >= std::fabs(x) ? (s - y) + x : (x - y) + s;
-                    s = y;
-                }
-                v[2 * k + c] = (s + comp) * l2;
-            }
-        }
+        for (size_t k = 0; k < 2 * (np + 1); k++) v[k] = ch.esum[si++] * l2;
         b->norm[e] = v[2 * np];
         v.resize(2 * np);
         b->expv[e] = std::move(v);
@@ -1803,12 +1871,15 @@ void run_batch(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, con
     auto flush = [&]() {
         if (cur.empty()) return;
         Chunk ch = make_chunk(b, cur, uniforms, draw_off, seeds, forced);
-        if (mode == 1)
+        if (mode == 1) {
             emulate_chunk(b, t, ch);
-        else if (mode == 2)
+            sum_partials(&ch);
+        } else if (mode == 2) {
             run_chunk_sim(b, t, ch);
-        else
+            sum_partials(&ch);
+        } else {
             run_chunk_device(b, ch);
+        }
         collect(b, ch);
         cur.clear();
         cur_bytes = 0;

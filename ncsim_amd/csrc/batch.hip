@@ -94,7 +94,31 @@ __global__ void k_batch_reset(uint32_t *done, uint32_t nrec, unsigned long long 
     }
 }
 
+// Each sum of per-tile partials, one thread per sum, in tile order with
+// Neumaier compensation (the same sequence as the host's sum_partials).
+__global__ void k_batch_finish(const double *__restrict__ part, const BSumD *__restrict__ sums, uint32_t count,
+                               double *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const BSumD r = sums[i];
+    const double *p = part + r.off;
+    double s = 0.0, comp = 0.0;
+    for (uint32_t t = 0; t < r.tiles; t++) {
+        const double x = p[(uint64_t)r.stride * t];
+        const double y = s + x;
+        comp += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
+        s = y;
+    }
+    out[i] = s + comp;
+}
+
 }  // namespace
+
+void launch_finish(const double *part, const BSumD *sums, uint32_t count, double *out, hipStream_t stream) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_batch_finish, dim3((count + 255) / 256), dim3(256), 0, stream, part, sums, count, out);
+    check_hip(hipGetLastError(), "k_batch_finish");
+}
 
 uint32_t max_tile_bits(ncsv_precision prec) { return prec == NCSV_C128 ? 12u : 13u; }
 

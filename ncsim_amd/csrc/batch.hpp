@@ -112,6 +112,13 @@ struct BRecD {
     uint32_t var;         // RF_ONE: the step's noise slot (index in its slot list) holding the one error
 };
 
+// One expectation sum of an entry (a Pauli's or the norm's real or imaginary
+// part): its per-tile partials sit at off + stride * t for t < tiles.
+struct BSumD {
+    uint64_t off;
+    uint32_t stride, tiles;
+};
+
 // RF_ONE (with RF_ERR): exactly one error was drawn in this step, so the op
 // list keeping only its slot runs instead of the list with every slot
 enum BRecFlags : uint32_t { RF_ERR = 1, RF_RELEASE = 2, RF_ONE = 4 };
