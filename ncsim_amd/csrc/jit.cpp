@@ -1307,25 +1307,27 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     };
     auto roff = [&](uint32_t r) { return roff_of(rpl, r); };
     if (from_basis) {
-        // a tile without the basis index starts at zero and stays zero (the
+        // A tile without the basis index starts at zero and stays zero (the
         // sweep's non-diagonal ops act on tile qubits only; its phases map 0
-        // to 0): its workgroup stores zeros and returns (a workgroup-uniform
-        // branch), so only the basis index's tile does the arithmetic
+        // to 0), so only the basis index's tile does arithmetic.  The zeros
+        // are written in address order instead of tile order: workgroup b
+        // clears the contiguous run of 2^L amplitudes from b * 2^L (one
+        // contiguous 2^LT run per store), skipping the basis tile's own
+        // amplitudes, which that tile's workgroup writes (a workgroup-uniform
+        // branch after the clearing).
         uint64_t outer = 0;
         for (uint32_t q : g.oq) outer |= 1ull << q;
-        o << "  if (base != (binit & " << outer << "ull)) {\n";
-        std::ostringstream e;
-        e << "base";
-        for (uint32_t i = 0; i < g.LT; i++)
-            e << " | (((unsigned long long)(tid >> " << i << ") & 1ull) << " << low_s[i] << ")";
-        o << "    const unsigned long long gz = " << e.str() << ";\n    V z; z.x = (T)0; z.y = (T)0;\n";
+        o << "  {\n    const unsigned long long cb = b << " << g.L << ", bo = binit & " << outer
+          << "ull;\n    V z; z.x = (T)0; z.y = (T)0;\n";
         for (uint32_t r = 0; r < g.NR; r++) {
+            o << "    { const unsigned long long i = cb | " << ((uint64_t)r << g.LT) << "ull | tid; if ((i & " << outer
+              << "ull) != bo) ";
             if (nt & 1)
-                o << "    st_nt(&amp[gz | " << roff_of(rps, r) << "ull], z);\n";
+                o << "st_nt(&amp[i], z); }\n";
             else
-                o << "    amp[gz | " << roff_of(rps, r) << "ull] = z;\n";
+                o << "amp[i] = z; }\n";
         }
-        o << "    return;\n  }\n";
+        o << "    if (base != bo) return;\n  }\n";
     }
     for (uint32_t r = 0; r < g.NR; r++) {
         if (from_basis)
