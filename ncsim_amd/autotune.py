@@ -13,8 +13,15 @@ against each other differently per workload and device:
   shorter ones spread a step over more workgroups.
 
 `tune_batch` times each candidate on a stratified sample of the batch's own
-entries (same circuits, same Paulis, seeded draws), applies the fastest to
-the batch (near-ties to the larger queue item) and records it in a JSON cache keyed by device, precision and the
+entries (same circuits, same Paulis, seeded draws) and takes the fastest
+(near-ties to the larger queue item).  Longer queue items pay off only when
+the whole batch keeps every workgroup busy, which a sample does not (the
+4096-circuit complex128 mix: 851 ms per run with 32 tiles per item, 797 ms
+with 64, 783 ms with 128, while a quarter of it runs fastest with 64 and an
+eighth with 32), so when the whole batch runs in about `full_budget_s` or
+less, the item sizes are then timed on the batch itself with the chosen tile
+shape (the results do not depend on the item size).  The winner is applied
+to the batch and recorded in a JSON cache keyed by device, precision and the
 batch's shape (qubit-count histogram, draws or none), so later batches of
 the same kind skip the search.
 """
@@ -78,9 +85,14 @@ def choose(timings):
     return max(near, key=lambda t: (t[0]["tiles_per_item"], -t[1]))[0]
 
 
-def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None):
+ITEMS = (32, 64, 128)  # queue item sizes timed on the whole batch
+
+
+def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None, full_budget_s=2.0):
     """Chooses and applies the fastest candidate; returns
-    {"config": ..., "timings": [(config, kernel_ms), ...], "cached": bool}."""
+    {"config": ..., "timings": [(config, kernel_ms), ...], "cached": bool}.
+    The item-size stage runs the batch itself (seeded draws), so results
+    read before a later run are those of a tuning run."""
     from . import HipBatch
 
     cache = cache or os.environ.get("NCSV_TUNE_CACHE", DEFAULT_CACHE)
@@ -118,6 +130,21 @@ def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None):
             log(f"autotune {cfg}: {best:.2f} ms")
         del b
     cfg = choose(timings)
+    est_s = min(t[1] for t in timings) * n / len(idx) / 1e3  # the whole batch at the sample's rate
+    if n > len(idx) and est_s <= full_budget_s:
+        whole = []
+        all_seeds = list(range(n))
+        for items in ITEMS:
+            c2 = dict(cfg, tiles_per_item=items)
+            _apply(batch, c2)
+            batch.run(seeds=all_seeds)  # plans once per tile shape, captures the launch
+            batch.run(seeds=all_seeds)
+            ms = batch.stats()["kernel_ms"]
+            whole.append((c2, ms))
+            if log:
+                log(f"autotune {c2} on the whole batch: {ms:.2f} ms")
+        cfg = min(whole, key=lambda t: t[1])[0]
+        timings += whole
     _apply(batch, cfg)
     db[key] = {"config": cfg, "timings": timings, "sample": len(idx), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
     try:

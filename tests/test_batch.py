@@ -583,15 +583,18 @@ def test_gpu_graph_replays_match_eager(monkeypatch):
 
 @pytest.mark.gpu
 def test_gpu_autotune_keeps_results(tmp_path):
-    """The autotuner times its candidates on a sample, applies one, caches
-    it; results stay those of the oracle."""
+    """The autotuner times its candidates on a sample, then the queue item
+    sizes on the whole batch, applies one, caches it; results stay those of
+    the oracle."""
+    from ncsim_amd import autotune as AT
     specs = [(n, 6, 300 + n) for n in (10, 12, 14, 15, 16)]
     texts = [random_circuit(n, d, s, noise=False) for n, d, s in specs]
     paulis = [random_paulis(n, 4, s) for n, d, s in specs]
     b = _build(texts, paulis, nc.C128, 0)
     cache = str(tmp_path / "tune.json")
-    r = b.autotune(sample=5, reps=1, cache=cache)
+    r = b.autotune(sample=3, reps=1, cache=cache)
     assert not r["cached"] and r["config"] in [c for c, _ in r["timings"]]
+    assert len(r["timings"]) == len(AT.candidates(nc.C128)) + len(AT.ITEMS)
     assert b.autotune(sample=5, reps=1, cache=cache)["cached"]
     us = _draws(b, 3)
     b.run(uniforms=us)
