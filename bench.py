@@ -220,6 +220,7 @@ def run_batch(args):
             b.add(circs[i], paulis[i])
         if args.tile_bits:
             b.set_tile_bits(args.tile_bits)
+            b.set_shape(args.low_bits, args.items)
         elif not args.no_autotune:
             t0 = time.perf_counter()
             r = b.autotune(log=log)
@@ -344,6 +345,7 @@ def run_traj(args):
     tuned = {}
     if args.tile_bits:
         b64.set_tile_bits(args.tile_bits)
+        b64.set_shape(args.low_bits, args.items)
     elif not args.no_autotune:
         t0 = time.perf_counter()
         r = b64.autotune(log=log, sample=48)
@@ -586,6 +588,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["c128", "c64"],
                     help="default: complex128 for the brickwork, complex64 for the batch")
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--low-bits", type=int, default=-1, help="batch workload with --tile-bits: fixed low tile qubits")
+    ap.add_argument("--items", type=int, default=-1, help="batch workload with --tile-bits: tiles per queue item")
     ap.add_argument("--qmin", type=int, default=0, help="batch workload: only circuits of at least this many qubits")
     ap.add_argument("--qmax", type=int, default=0, help="batch workload: only circuits of at most this many qubits")
     ap.add_argument("--no-autotune", action="store_true")
