@@ -1839,15 +1839,26 @@ bool available() {
 // so more X parts share a stage) by 13 (five passes of 2.4 ms against six of
 // 2.2 ms at 12 and five of 3.0 ms at 16, where 16 Paulis take the kernel to
 // 256 registers and one wavefront per SIMD).
+// Complex128 groups of 13 are taken only where they save a pass (with three
+// fixed low qubits, so nine free tile positions, and the order search below,
+// the 64 Paulis fit five groups of 13: 5 x 3.38 ms against 6 x 3.06 ms; where
+// they save none, 13 cost 2.6 % per pass).
 uint32_t group_max(const TileGeom &geo) {
     const char *e = getenv("NCSV_EXPECT_GMAX");
     if (e) return (uint32_t)std::min(16, std::max(1, atoi(e)));
     return geo.R >= 5 ? 13u : 12u;
 }
 
-bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
-                 const uint32_t *ph, std::vector<ExpGroup> *out_groups) {
-    const uint32_t gmax = group_max(geo);
+// Orders of the first fit tried: the X-weight order, plus (from 26 qubits,
+// where a pass over the state costs a fraction of a millisecond or more) 511
+// orders with the weights jittered; NCSV_EXPECT_ORDERS sets the count.
+uint32_t expect_orders(uint32_t n) {
+    if (const char *e = getenv("NCSV_EXPECT_ORDERS")) return (uint32_t)std::max(1, atoi(e));
+    return n >= 26 ? 512u : 1u;
+}
+
+bool plan_expect_g(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
+                   const uint32_t *ph, uint32_t gmax, std::vector<ExpGroup> *out_groups) {
     const uint32_t L = geo.L(), R = geo.R, c = std::min(geo.c, L), LT = geo.LT;
     if (n < L) return false;
     ncsv_state fake;
@@ -1862,24 +1873,47 @@ bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t
         uint64_t T;
         std::vector<uint32_t> idx;
     };
-    std::vector<G0> g0;
     // first fit, widest X parts first (fewer tiles, so fewer passes over the state)
+    auto first_fit = [&](const std::vector<uint32_t> &order) {
+        std::vector<G0> g0;
+        for (uint32_t i : order) {
+            const uint64_t need = x[i] & ~low;
+            bool placed = false;
+            for (auto &g : g0)
+                if (g.idx.size() < gmax && (uint32_t)__builtin_popcountll(g.T | need) <= L) {
+                    g.T |= need;
+                    g.idx.push_back(i);
+                    placed = true;
+                    break;
+                }
+            if (!placed) g0.push_back({low | need, {i}});
+        }
+        return g0;
+    };
     std::vector<uint32_t> order(count);
     for (uint32_t i = 0; i < count; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return __builtin_popcountll(x[a] & ~low) > __builtin_popcountll(x[b] & ~low);
     });
-    for (uint32_t i : order) {
-        const uint64_t need = x[i] & ~low;
-        bool placed = false;
-        for (auto &g : g0)
-            if (g.idx.size() < gmax && (uint32_t)__builtin_popcountll(g.T | need) <= L) {
-                g.T |= need;
-                g.idx.push_back(i);
-                placed = true;
-                break;
-            }
-        if (!placed) g0.push_back({low | need, {i}});
+    std::vector<G0> g0 = first_fit(order);
+    // more orders (X weight plus a fixed pseudo-random jitter of up to 2.5),
+    // keeping the first with the fewest groups
+    const uint32_t orders = expect_orders(n);
+    const size_t fewest = (count + gmax - 1) / gmax;
+    for (uint32_t t = 1; t < orders && g0.size() > fewest; t++) {
+        std::vector<double> key(count);
+        for (uint32_t i = 0; i < count; i++) {
+            uint64_t h = ((uint64_t)t << 32 | i) * 0x9E3779B97F4A7C15ull;
+            h ^= h >> 29;
+            h *= 0xBF58476D1CE4E5B9ull;
+            h ^= h >> 32;
+            key[i] = -(double)__builtin_popcountll(x[i] & ~low) + 2.5 * (double)(h >> 11) / 9007199254740992.0;
+        }
+        std::vector<uint32_t> o2(count);
+        for (uint32_t i = 0; i < count; i++) o2[i] = i;
+        std::stable_sort(o2.begin(), o2.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+        std::vector<G0> g2 = first_fit(o2);
+        if (g2.size() < g0.size()) g0 = std::move(g2);
     }
     std::vector<ExpGroup> groups;
     for (auto &g : g0) {
@@ -1941,9 +1975,56 @@ bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t
     return true;
 }
 
+// The groups of a Pauli list, planned once per list, tile shape and state
+// size (the order search makes planning cost up to a millisecond).
+bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
+                 const uint32_t *ph, std::vector<ExpGroup> *out_groups) {
+    static std::mutex mu;
+    static std::map<std::string, std::vector<ExpGroup>> cache;
+    std::string key;
+    {
+        const uint32_t head[7] = {n, geo.LT, geo.R, geo.c, count, group_max(geo), expect_orders(n)};
+        key.append((const char *)head, sizeof head);
+        key.append((const char *)x, count * sizeof(uint64_t));
+        key.append((const char *)z, count * sizeof(uint64_t));
+        key.append((const char *)ph, count * sizeof(uint32_t));
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            *out_groups = it->second;
+            return true;
+        }
+    }
+    const uint32_t gmax = group_max(geo);
+    std::vector<ExpGroup> groups;
+    if (!plan_expect_g(n, geo, count, x, z, ph, gmax, &groups)) return false;
+    if (gmax == 12 && geo.R < 5 && !getenv("NCSV_EXPECT_GMAX") && groups.size() > 1) {
+        std::vector<ExpGroup> g13;
+        if (plan_expect_g(n, geo, count, x, z, ph, 13, &g13) && g13.size() < groups.size()) groups = std::move(g13);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 256) cache.clear();
+    cache.emplace(key, groups);
+    *out_groups = std::move(groups);
+    return true;
+}
+
+// The expectation kernels' tile shape: the sweeps' default, but complex128
+// tiles fix three low qubits instead of four (128-byte runs, the same pass
+// time: 3.05 against 3.06 ms at 30 qubits), leaving nine free positions for
+// the groups' X qubits; NCSV_EXPECT_C sets the fixed low qubits.
+TileGeom expect_geom(ncsv_precision prec) {
+    TileGeom geo = dev::tile_geom(prec);
+    if (prec == NCSV_C128 && !getenv("NCSV_TILE_C")) geo.c = 3;
+    if (const char *e = getenv("NCSV_EXPECT_C")) geo.c = (uint32_t)atoi(e);
+    return geo;
+}
+
 bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
                   double *out) {
-    const TileGeom geo = dev::tile_geom(s->prec);
+    const TileGeom geo = expect_geom(s->prec);
     const uint32_t L = geo.L(), LT = geo.LT;
     constexpr uint32_t GMAX = 16;
     std::vector<ExpGroup> groups;
@@ -1984,7 +2065,7 @@ ncsv_status ncsv_jit_check_expect(uint32_t n, ncsv_precision prec, uint32_t coun
                                   const uint64_t *z, const uint32_t *ph, const char *arch, uint32_t *ngroups) {
     try {
         using namespace ncsv;
-        const TileGeom geo = dev::tile_geom(prec);
+        const TileGeom geo = jit::expect_geom(prec);
         std::vector<jit::ExpGroup> groups;
         if (!jit::plan_expect(n, geo, count, x, z, ph, &groups)) return NCSV_UNSUPPORTED;
         for (auto &eg : groups) {

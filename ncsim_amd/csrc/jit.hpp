@@ -64,6 +64,8 @@ void launch_expect(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, uint32_
 // does not fit a tile's register stage (nothing done then).
 bool plan_expect(uint32_t n, const TileGeom &geo, uint32_t count, const uint64_t *x, const uint64_t *z,
                  const uint32_t *ph, std::vector<ExpGroup> *out_groups);
+// the expectation kernels' tile shape
+TileGeom expect_geom(ncsv_precision prec);
 bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64_t *z, const uint32_t *ph,
                   double *out);
 }  // namespace jit
