@@ -463,19 +463,31 @@ std::vector<BOpD> fuse_blocks(StepH *s, const std::vector<BOpD> &in, bool fp32_c
             }
         d.coef = (uint32_t)s->coef.size();
         s->coef.insert(s->coef.end(), M.begin(), M.end());
-        // complex64: the kernel's FP32 copy per variant (four floats per slot: row r,
-        // columns 2q and 2q + 1)
+        // the kernel's A operands per variant, in lane order: lane l (i = l % 16,
+        // q = l / 16) holds the real-form entries of row ai (U[ai][2q], U[ai][2q + 1])
+        // that its four MFMAs take, in the operand's precision (complex64: one
+        // 16-byte slot per lane; complex128: two), so the kernel loads them as they
+        // are and its first MFMA is the first use of the load
         d.a = (uint32_t)s->coef.size();
-        for (uint32_t var = 0; var < nvar && fp32_copy; var++)
-            for (uint32_t r = 0; r < 8; r++)
-                for (uint32_t q = 0; q < 4; q++) {
-                    const cd *m = &M[64 * var + 8 * r + 2 * q];
-                    const float f[4] = {(float)m[0].real(), (float)m[0].imag(), (float)m[1].real(), (float)m[1].imag()};
+        for (uint32_t var = 0; var < nvar; var++)
+            for (uint32_t l = 0; l < 64; l++) {
+                const uint32_t i = l & 15, q = l >> 4;
+                const uint32_t ai = fp32_copy ? 2 * (i >> 2) + ((i >> 1) & 1) : 2 * (i & 3) + (i >> 3);
+                const bool ci = fp32_copy ? (i & 1) != 0 : ((i >> 2) & 1) != 0;
+                const cd u0 = M[64 * var + 8 * ai + 2 * q], u1 = M[64 * var + 8 * ai + 2 * q + 1];
+                const double av[4] = {ci ? u0.imag() : u0.real(), ci ? u0.real() : -u0.imag(),
+                                      ci ? u1.imag() : u1.real(), ci ? u1.real() : -u1.imag()};
+                if (fp32_copy) {
+                    const float f[4] = {(float)av[0], (float)av[1], (float)av[2], (float)av[3]};
                     cd slot;
                     static_assert(sizeof(slot) == sizeof(f), "four floats per complex slot");
                     memcpy((void *)&slot, f, sizeof(f));
                     s->coef.push_back(slot);
+                } else {
+                    s->coef.push_back(cd(av[0], av[1]));
+                    s->coef.push_back(cd(av[2], av[3]));
                 }
+            }
         out.push_back(d);
     };
     auto close_if = [&](uint32_t m) {

@@ -439,32 +439,16 @@ NCSV_KERNEL void k_batch(const BStepD *__restrict__ steps, const BOpD *__restric
                     // receives amplitudes 2q and 2q + 1 of its group and nothing else.
                     // A[i][k] sits in lane i + 16 (k % 4), register k / 4 (both sizes); the
                     // row i of output register v in lane l is 4 q + v (FP32) or 4 v + q
-                    // (FP64), so A's rows are ordered to match.
+                    // (FP64), so A's rows are ordered to match.  The planner lays each
+                    // lane's four A values out at op.a (per variant, in lane order).
                     const uint32_t pb[3] = {1u << op.p0, 1u << op.p1, 1u << op.p2};
                     const uint32_t mask = pb[0] | pb[1] | pb[2];
                     const uint32_t lane = tid & 63, q = lane >> 4, wv = tid >> 6;
-                    const uint32_t i = lane & 15;
-                    const uint32_t ai = sizeof(T) == 4 ? 2 * (i >> 2) + ((i >> 1) & 1) : 2 * (i & 3) + (i >> 3);
-                    const uint32_t ci = sizeof(T) == 4 ? (i & 1) : ((i >> 2) & 1);
                     // two matrices when ops of the block are conditioned on outer qubits:
                     // the second for tiles that meet the condition
                     const uint32_t var = (op.omask && (base & (uint32_t)op.omask) == (uint32_t)op.omask) ? 1u : 0u;
-                    T u0r, u0i, u1r, u1i;  // U[ai][2q], U[ai][2q + 1]
-                    if constexpr (sizeof(T) == 4) {
-                        // the FP32 copy of the block at op.a: row-major complex pairs
-                        const float *ur = (const float *)(cf + 2 * (op.a + 32 * var)) + 16 * ai + 4 * q;
-                        u0r = ur[0];
-                        u0i = ur[1];
-                        u1r = ur[2];
-                        u1i = ur[3];
-                    } else {
-                        const double *ur = cf + 2 * (op.coef + 64 * var + 8 * ai + 2 * q);
-                        u0r = ur[0];
-                        u0i = ur[1];
-                        u1r = ur[2];
-                        u1i = ur[3];
-                    }
-                    const T av[4] = {ci ? u0i : u0r, ci ? u0r : -u0i, ci ? u1i : u1r, ci ? u1r : -u1i};
+                    const T *ap = (const T *)(cf + 2 * op.a) + 4 * (64 * var + lane);
+                    const T av[4] = {ap[0], ap[1], ap[2], ap[3]};
                     auto off = [&](uint32_t a) {
                         return sw<V>(((a & 1) ? pb[0] : 0u) | ((a & 2) ? pb[1] : 0u) | ((a & 4) ? pb[2] : 0u));
                     };
