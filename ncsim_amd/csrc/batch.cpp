@@ -72,6 +72,10 @@ struct Program {
     // placement in the device tables
     uint32_t step0 = 0;
     uint32_t tiles(uint32_t s) const { return 1u << (n - steps[s].L); }
+    // per step, what each run's queue records need (kept apart from the step
+    // lists, which the host would otherwise walk once per record): log2 of
+    // the tiles, and whether the step holds noise slots
+    std::vector<uint8_t> step_lg, step_noisy;
 };
 
 namespace {
@@ -976,6 +980,8 @@ std::shared_ptr<Program> build_program(const ncsv_circuit *c, const std::vector<
         fputs(txt.c_str(), stderr);
     }
     for (uint32_t s = 0; s < P->steps.size(); s++) {
+        P->step_lg.push_back((uint8_t)(n - P->steps[s].L));
+        P->step_noisy.push_back(!P->steps[s].slots.empty());
         P->items += P->tiles(s);
         if (n > Lb) {
             const uint64_t N = 1ull << n;
@@ -1378,36 +1384,46 @@ Chunk make_chunk(ncsv_batch *b, const std::vector<uint32_t> &ent, const double *
                    forced ? forced + draw_off[e] : nullptr, &ch, ch.ed[i]);
     }
     // depth-major queue, deeper programs first within a depth
-    std::vector<uint32_t> order(ne);
-    for (uint32_t i = 0; i < ne; i++) order[i] = i;
-    auto nsteps = [&](uint32_t i) { return (uint32_t)b->progs[b->entries[ent[i]].prog]->steps.size(); };
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return nsteps(a) > nsteps(c); });
+    std::vector<const Program *> prog(ne);
+    std::vector<uint32_t> nst(ne), order(ne);
+    size_t nrec = 0;
+    for (uint32_t i = 0; i < ne; i++) {
+        prog[i] = b->progs[b->entries[ent[i]].prog].get();
+        nst[i] = (uint32_t)prog[i]->steps.size();
+        nrec += nst[i];
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return nst[a] > nst[c]; });
     std::vector<uint32_t> last(ne, ~0u);
-    const uint32_t maxd = ne ? nsteps(order[0]) : 0;
+    ch.recs.reserve(nrec + 1);
+    const uint32_t maxd = ne ? nst[order[0]] : 0;
     for (uint32_t d = 0; d < maxd; d++) {
         for (uint32_t i : order) {
-            const Program &P = *b->progs[b->entries[ent[i]].prog];
-            if (d >= P.steps.size()) break;  // sorted by depth
+            const Program &P = *prog[i];
+            if (d >= nst[i]) break;  // sorted by depth
             BRecD r{};
             r.first = ch.total;
             r.entry = i;
             r.step = P.step0 + d;
             r.prev = last[i];
-            r.prev_tiles = d ? P.tiles(d - 1) : 0;
-            uint32_t err = 0;
-            for (uint32_t sl : P.steps[d].slots) err |= ch.errs[ch.ed[i].err0 + sl];
-            r.flags = (err ? RF_ERR : 0u) | (d + 1 < P.steps.size() ? RF_RELEASE : 0u);
-            const int one = single_error(P.steps[d], &ch.errs[ch.ed[i].err0]);
-            if (one >= 0) {
-                r.flags |= RF_ONE;
-                r.var = (uint32_t)one;
+            r.prev_tiles = d ? 1u << P.step_lg[d - 1] : 0;
+            r.flags = d + 1 < nst[i] ? RF_RELEASE : 0u;
+            if (P.step_noisy[d]) {
+                uint32_t err = 0;
+                for (uint32_t sl : P.steps[d].slots) err |= ch.errs[ch.ed[i].err0 + sl];
+                if (err) r.flags |= RF_ERR;
+                const int one = single_error(P.steps[d], &ch.errs[ch.ed[i].err0]);
+                if (one >= 0) {
+                    r.flags |= RF_ONE;
+                    r.var = (uint32_t)one;
+                }
             }
             // at least 8 items per step where the step has that many tiles
-            const uint32_t lg = P.n - P.steps[d].L;
+            const uint32_t lg = P.step_lg[d];
             r.lgt = std::min(lg > 3 ? lg - 3 : 0u, lg_tiles_per_item);
             last[i] = (uint32_t)ch.recs.size();
             ch.recs.push_back(r);
-            ch.total += P.tiles(d) >> r.lgt;
+            ch.total += (1ull << lg) >> r.lgt;
         }
     }
     if (getenv("NCSV_BATCH_DUMP")) {
@@ -1880,9 +1896,13 @@ void run_batch(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, con
     }
     std::vector<uint32_t> cur;
     size_t cur_bytes = 0;
+    // NCSV_BATCH_TIMING=1: the host's share of each launch on stderr
+    static const bool timing = getenv("NCSV_BATCH_TIMING") != nullptr;
     auto flush = [&]() {
         if (cur.empty()) return;
+        const auto t0 = std::chrono::steady_clock::now();
         Chunk ch = make_chunk(b, cur, uniforms, draw_off, seeds, forced);
+        const auto t1 = std::chrono::steady_clock::now();
         if (mode == 1) {
             emulate_chunk(b, t, ch);
             sum_partials(&ch);
@@ -1892,7 +1912,14 @@ void run_batch(ncsv_batch *b, const double *uniforms, const uint64_t *seeds, con
         } else {
             run_chunk_device(b, ch);
         }
+        const auto t2 = std::chrono::steady_clock::now();
         collect(b, ch);
+        const auto t3 = std::chrono::steady_clock::now();
+        if (timing) {
+            auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
+            fprintf(stderr, "batch launch: %zu entries, chunk %.2f ms, run %.2f ms (kernel %.2f ms), collect %.2f ms\n",
+                    cur.size(), ms(t0, t1), ms(t1, t2), b->kernel_ms, ms(t2, t3));
+        }
         cur.clear();
         cur_bytes = 0;
     };
