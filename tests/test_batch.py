@@ -582,6 +582,32 @@ def test_gpu_graph_replays_match_eager(monkeypatch):
 
 
 @pytest.mark.gpu
+def test_gpu_queue_spread_and_item_sizes_keep_results(monkeypatch):
+    """Item sizes change only the queue: the thin depths cut into shorter
+    items (a low spread so that this small batch qualifies), the default
+    queue and long items give bit-identical results, equal to the oracle."""
+    specs = [(n, d, 900 + k) for k, (n, d) in enumerate([(14, 9), (15, 14), (16, 6), (13, 20), (15, 4), (16, 11)])]
+    texts = [random_circuit(n, d, s) for n, d, s in specs]
+    paulis = [random_paulis(n, 3, s) for n, d, s in specs]
+    b = _build(texts, paulis, nc.C128, 8)
+    us = _draws(b, 5)
+    runs = []
+    for spread, items in (("0", -1), ("16", -1), ("16", 128), ("0", 1)):
+        monkeypatch.setenv("NCSV_BATCH_SPREAD", spread)
+        b.set_shape(-1, items)
+        b.run(uniforms=us)
+        runs.append((b.stats()["items"], [b.result(i) for i in range(len(b))]))
+    assert runs[1][0] > runs[0][0]  # the spread cut some depths finer
+    for _, res in runs[1:]:
+        assert res == runs[0][1]
+    for i, (t, ps) in enumerate(zip(texts, paulis)):
+        ex, nn, oc = runs[0][1][i]
+        rex, rnn, roc = oracle_entry(t, ps, us[i])
+        assert oc == roc
+        assert max(abs(a - r) for a, r in zip(ex, rex)) < 1e-12
+
+
+@pytest.mark.gpu
 def test_gpu_autotune_keeps_results(tmp_path):
     """The autotuner times its candidates on a sample, then the queue item
     sizes on the whole batch, applies one, caches it; results stay those of
