@@ -95,6 +95,7 @@ void free_state(ncsv_state *s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->amp) (void)hipFree(s->amp);
+    if (s->spare) (void)hipFree(s->spare);
     if (s->d_part) (void)hipFree(s->d_part);
     if (s->d_tab) (void)hipFree(s->d_tab);
     if (s->d_samp) (void)hipFree(s->d_samp);

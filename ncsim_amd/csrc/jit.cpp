@@ -1272,9 +1272,12 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     o.str("");
     o << "extern \"C\" __global__ __launch_bounds__(" << (1u << g.LT);
     if (waves) o << ", " << waves;
-    o << ") void " << (from_basis ? "ncsv_sweep_from_basis" : "ncsv_sweep")
-      << "(V *__restrict__ amp, const T *__restrict__ c" << (from_basis ? ", unsigned long long binit" : "")
-      << ") {\n";
+    // a regular sweep reads amp and writes out (the same buffer, or the
+    // state's second one when it writes out of place: no restrict on either)
+    if (from_basis)
+        o << ") void ncsv_sweep_from_basis(V *__restrict__ amp, const T *__restrict__ c, unsigned long long binit) {\n";
+    else
+        o << ") void ncsv_sweep(V *amp, const T *__restrict__ c, V *out) {\n";
     if (lds) o << "  __shared__ V lds[" << (1u << (g.half ? g.L - 1 : g.L)) << "];\n";
     o << "  const unsigned tid = threadIdx.x;\n";
     if (border == 1 && nb >= 3)
@@ -1390,9 +1393,9 @@ std::string generate(const SweepH &sw, uint32_t n, const TileGeom &geo, bool dbl
     }
     for (uint32_t r = 0; r < g.NR; r++) {
         if (nt & 1)
-            o << "  st_nt(&amp[gs | " << roff_of(rps, r) << "ull], " << g.var(r) << ");\n";
+            o << "  st_nt(&" << (from_basis ? "amp" : "out") << "[gs | " << roff_of(rps, r) << "ull], " << g.var(r) << ");\n";
         else
-            o << "  amp[gs | " << roff_of(rps, r) << "ull] = " << g.var(r) << ";\n";
+            o << "  " << (from_basis ? "amp" : "out") << "[gs | " << roff_of(rps, r) << "ull] = " << g.var(r) << ";\n";
     }
     o << "}\n";
     {
@@ -1779,17 +1782,37 @@ void *prepare_named(const std::string &src, const char *fname) {
 
 void release(void *handle) { delete (std::shared_future<std::shared_ptr<Entry>> *)handle; }
 
+// NCSV_JIT_OOP=1: sweeps read one buffer and write the other (the state's
+// second buffer, allocated on first use; in place when it cannot be had).
+bool out_of_place() {
+    static const bool v = getenv("NCSV_JIT_OOP") && atoi(getenv("NCSV_JIT_OOP")) != 0;
+    return v;
+}
+
 void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef) {
     auto &f = *(std::shared_future<std::shared_ptr<Entry>> *)handle;
     const std::shared_ptr<Entry> e = f.get();
     if (!e->fn) fail(NCSV_DEVICE, e->err);
+    if (out_of_place() && !s->spare && !s->spare_failed) {
+        bool ok = true;
+        if (const char *cap = getenv("NCSV_DEVICE_MEMORY_LIMIT_GB"))
+            ok = 2.0 * (double)s->amp_bytes() <= atof(cap) * (double)(1ull << 30);
+        if (ok && hipMalloc(&s->spare, s->amp_bytes()) != hipSuccess) {
+            (void)hipGetLastError();  // the refused allocation's error is not the caller's
+            s->spare = nullptr;
+            ok = false;
+        }
+        s->spare_failed = !ok;
+    }
     void *amp = s->amp;
+    void *out = s->spare ? s->spare : s->amp;
     const void *c = dcoef;
-    void *args[] = {&amp, &c};
+    void *args[] = {&amp, &c, &out};
     const uint64_t tiles = 1ull << (s->n - L);
     KTimer kt(s, "sweep_jit", 2.0 * (double)s->amp_bytes());
     check_hip(hipModuleLaunchKernel(e->fn, (uint32_t)tiles, 1, 1, 1u << LT, 1, 1, 0, s->stream, args, nullptr),
               "hipModuleLaunchKernel(sweep)");
+    if (out != amp) std::swap(s->amp, s->spare);
     s->sweeps++;
 }
 

@@ -127,6 +127,11 @@ struct ncsv_state {
     ncsv_precision prec = NCSV_C128;
     int device = 0;
     void *amp = nullptr;  // 2^n complex amplitudes (double2 or float2)
+    // a second buffer of the same size for sweeps that write out of place
+    // (allocated on the first such sweep; the two swap after each of them);
+    // spare_failed: the allocation was refused, sweeps stay in place
+    void *spare = nullptr;
+    bool spare_failed = false;
     double lscale = 1.0;  // logical amplitude = lscale * stored (complex64 form)
     // set_basis deferred: the state is |pending_basis> but the amplitudes are
     // not written yet (-1: they are); a run's first specialized sweep computes

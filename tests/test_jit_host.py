@@ -88,7 +88,17 @@ extern "C" void run_host(V *amp, unsigned long long nblocks, unsigned nthreads, 
         g_bar = &bar;
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nthreads; t++)
-            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_sweep(amp, coef); });
+            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_sweep(amp, coef, amp); });
+        for (auto &x : th) x.join();
+    }
+}
+extern "C" void run_host_oop(V *in, V *out, unsigned long long nblocks, unsigned nthreads, const T *coef) {
+    for (unsigned long long b = 0; b < nblocks; b++) {
+        std::barrier<> bar(nthreads);
+        g_bar = &bar;
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nthreads; t++)
+            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_sweep(in, coef, out); });
         for (auto &x : th) x.join();
     }
 }
@@ -119,9 +129,10 @@ def host_kernel(src, workdir, name, post=None):
     return ctypes.CDLL(so)
 
 
-def run_plan_on_host(text, precision, monkeypatch, half, env=None):
+def run_plan_on_host(text, precision, monkeypatch, half, env=None, oop=False):
     """Every sweep kernel of the circuit's plan, in order, on the host;
-    returns (host result, emulated result, plan stats)."""
+    returns (host result, emulated result, plan stats).  oop: each sweep
+    reads one buffer and writes another that starts as NaN."""
     work = tempfile.mkdtemp()
     try:
         monkeypatch.setenv("NCSV_JIT_DUMP", work)
@@ -148,8 +159,15 @@ def run_plan_on_host(text, precision, monkeypatch, half, env=None):
             coef = np.fromfile(os.path.join(work, f"ncsv_{h}.coef"), dtype=np.float64)
             coef = np.ascontiguousarray(coef if precision == H.C128 else coef.astype(np.float32))
             lib = host_kernel(src, work, f"k{i}")
-            lib.run_host(buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_ulonglong(1 << (n - tile)), ctypes.c_uint(256),
-                         coef.ctypes.data_as(ctypes.c_void_p))
+            if oop:
+                out = np.full_like(buf, np.nan)
+                lib.run_host_oop(buf.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.c_ulonglong(1 << (n - tile)), ctypes.c_uint(256),
+                                 coef.ctypes.data_as(ctypes.c_void_p))
+                buf = out
+            else:
+                lib.run_host(buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_ulonglong(1 << (n - tile)),
+                             ctypes.c_uint(256), coef.ctypes.data_as(ctypes.c_void_p))
         return buf.astype(np.complex128), want, list(stats)
     finally:
         shutil.rmtree(work, ignore_errors=True)
@@ -191,6 +209,18 @@ def test_generated_sweeps_on_host_threads(name, precision, make, half, monkeypat
     got, want, stats = run_plan_on_host(make(), precision, monkeypatch, half)
     tol = 1e-13 if precision == H.C128 else 2e-6
     assert np.abs(got - want).max() < tol, (name, stats)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ with C++20")
+@pytest.mark.parametrize("precision,make", [(H.C128, lambda: outer_heavy(16, 12, 4)),
+                                            (H.C64, lambda: W.rotation_brickwork(17, 4, 3))], ids=["c128", "c64"])
+def test_sweeps_out_of_place_on_host_threads(precision, make, monkeypatch):
+    """Each sweep reading one buffer and writing another (initialised to NaN)
+    writes every amplitude and computes the in-place result."""
+    got, want, stats = run_plan_on_host(make(), precision, monkeypatch, 1, oop=True)
+    assert not np.isnan(got).any()
+    tol = 1e-13 if precision == H.C128 else 2e-6
+    assert np.abs(got - want).max() < tol, stats
 
 
 # memory shapes: loads and stores in other register layouts (the candidates
