@@ -1785,8 +1785,8 @@ void release(void *handle) { delete (std::shared_future<std::shared_ptr<Entry>> 
 // NCSV_JIT_OOP=1: sweeps read one buffer and write the other (the state's
 // second buffer, allocated on first use; in place when it cannot be had).
 bool out_of_place() {
-    static const bool v = getenv("NCSV_JIT_OOP") && atoi(getenv("NCSV_JIT_OOP")) != 0;
-    return v;
+    const char *e = getenv("NCSV_JIT_OOP");
+    return e && atoi(e) != 0;
 }
 
 void launch(ncsv_state *s, void *handle, uint32_t L, uint32_t LT, const void *dcoef) {

@@ -291,6 +291,27 @@ def run_both(text, seed, fusion=None):
 
 
 @pytest.mark.parametrize("precision", [H.C128, H.C64])
+def test_out_of_place_sweeps_equal_in_place(precision, monkeypatch):
+    """NCSV_JIT_OOP=1: the specialized sweeps write a second buffer and the
+    two swap after each sweep; states, expectations and a second run on the
+    swapped buffers equal the in-place runs bit for bit."""
+    n = 22
+    circ = H.Circuit.parse(W.rotation_brickwork(n, 8, 11))
+    ps = [H.Pauli(n, x, z, ph) for x, z, ph in W.random_paulis(n, 16, 12)]
+    res = []
+    for oop in ("0", "1"):
+        monkeypatch.setenv("NCSV_JIT_OOP", oop)
+        d = H.HipStateVector.zero_state(n, precision)
+        d.run(circ)
+        v1 = d.to_vector()
+        d.run(circ)  # again, from the buffer the first run ended in
+        res.append((v1, d.to_vector(), d.expectations(ps)))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    assert res[0][2] == res[1][2]
+
+
+@pytest.mark.parametrize("precision", [H.C128, H.C64])
 def test_deferred_basis_state(precision):
     """set_basis defers its fill: a run's first specialized sweep (22 qubits)
     starts from the basis state in registers, and any other reader writes the
