@@ -447,14 +447,11 @@ def test_kernel_code_on_host_matches_oracle(precision, threads, tile_bits, monke
     or 256 threads with real barriers): whole-state entries and multi-tile
     entries whose steps wait on each other, on tiles below the thread bits
     and on the widest tile each workgroup size runs (chunked tile loads,
-    paired MFMA rounds, and in the 256-thread builds queue items of two
-    tiles)."""
+    paired MFMA rounds)."""
     monkeypatch.setenv("NCSV_BATCH_THREADS", threads)
     cases = [(5, 4, 31), (8, 5, 32), (9, 4, 33)]
     if tile_bits > 7:
         cases += [(tile_bits + 1, 4, 34), (tile_bits, 3, 35)]
-    if tile_bits > 7 and threads == "256":
-        cases += [(tile_bits + 4, 3, 36)]
     texts = [random_circuit(n, d, s) for n, d, s in cases]
     paulis = [random_paulis(n, 3, s) for n, d, s in cases]
     b = _build(texts, paulis, precision, tile_bits)

@@ -282,7 +282,7 @@ def su4_circuit(n, depth, seed):
 
 MFMA_CASES = [
     ("su4-auto", lambda: su4_circuit(14, 6, 1), {"NCSV_JIT_MFMA": "1"}),
-    ("su4-multi-tile", lambda: su4_circuit(16, 4, 2), {"NCSV_JIT_MFMA": "2"}),
+    ("su4-multi-tile", lambda: su4_circuit(15, 4, 2), {"NCSV_JIT_MFMA": "2"}),
     ("brickwork-forced", lambda: W.rotation_brickwork(14, 3, 3), {"NCSV_JIT_MFMA": "2"}),
     ("qft-forced", lambda: W.qft_with_rz(14, 8, 2), {"NCSV_JIT_MFMA": "2"}),
     ("outer-forced", lambda: outer_heavy(14, 13, 5), {"NCSV_JIT_MFMA": "2"}),
