@@ -26,11 +26,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ncsv_internal.hpp"
@@ -413,11 +415,18 @@ struct ncsv_frames {
     size_t buf_words = 0;
     hipStream_t stream = nullptr;
     bool ready = false;                  // device copies uploaded
+    // pinned staging for large downloads into the caller's memory
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t staged[2] = {nullptr, nullptr};
     ~ncsv_frames() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (d_ops) (void)hipFree(d_ops);
         if (d_hinit) (void)hipFree(d_hinit);
         if (d_buf) (void)hipFree(d_buf);
+        for (int k = 0; k < 2; k++) {
+            if (stage[k]) (void)hipHostFree(stage[k]);
+            if (staged[k]) (void)hipEventDestroy(staged[k]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -629,6 +638,49 @@ uint64_t *sample_device(ncsv_frames *f, uint64_t shots, uint64_t seed) {
     return rec;
 }
 
+// A device-to-host copy into the caller's (pageable) memory, done when it
+// returns.  Large ones go through two pinned staging buffers: one chunk
+// copies in while host threads spread the previous one into the destination
+// (a pageable destination is otherwise copied a page-locked piece at a time).
+void download(ncsv_frames *f, void *dst, const void *src, size_t bytes) {
+    constexpr size_t CH = size_t(32) << 20;
+    const char *e = getenv("NCSV_FRAMES_STAGE");  // 0: one plain copy (for comparison)
+    if (bytes < 4 * CH || (e && e[0] == '0')) {
+        check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, f->stream), "download");
+        check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
+        return;
+    }
+    for (int k = 0; k < 2; k++) {
+        if (!f->stage[k]) check_hip(hipHostMalloc(&f->stage[k], CH, hipHostMallocDefault), "hipHostMalloc(staging)");
+        if (!f->staged[k]) check_hip(hipEventCreateWithFlags(&f->staged[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    const unsigned T = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    auto spread = [&](size_t c) {
+        const size_t off = c * CH, len = std::min(CH, bytes - off), part = (len + T - 1) / T;
+        const char *from = (const char *)f->stage[c & 1];
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; t++) {
+            const size_t a = std::min(len, t * part), b = std::min(len, a + part);
+            if (a < b) th.emplace_back([=]() { memcpy((char *)dst + off + a, from + a, b - a); });
+        }
+        for (auto &x : th) x.join();
+    };
+    const size_t nch = (bytes + CH - 1) / CH;
+    for (size_t c = 0; c < nch; c++) {
+        const size_t off = c * CH, len = std::min(CH, bytes - off);
+        // stage c & 1 was last spread in the iteration before the previous one
+        check_hip(hipMemcpyAsync(f->stage[c & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, f->stream),
+                  "download");
+        check_hip(hipEventRecord(f->staged[c & 1], f->stream), "hipEventRecord");
+        if (c > 0) {
+            check_hip(hipEventSynchronize(f->staged[(c - 1) & 1]), "hipEventSynchronize");
+            spread(c - 1);
+        }
+    }
+    check_hip(hipEventSynchronize(f->staged[(nch - 1) & 1]), "hipEventSynchronize");
+    spread(nch - 1);
+}
+
 void parities_device(ncsv_frames *f, const uint64_t *rec, uint64_t shots, const std::vector<std::vector<uint32_t>> &sets,
                      uint64_t *host_out) {
     if (sets.empty()) return;
@@ -659,7 +711,7 @@ void parities_device(ncsv_frames *f, const uint64_t *rec, uint64_t shots, const 
             hipLaunchKernelGGL(k_parities, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, f->stream, rec, W, shots,
                                d_off, d_idx, d_flip, (uint32_t)sets.size(), d_out);
             check_hip(hipGetLastError(), "k_parities");
-            check_hip(hipMemcpyAsync(host_out, d_out, items * 8, hipMemcpyDeviceToHost, f->stream), "download");
+            download(f, host_out, d_out, items * 8);
         }
         check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
     } catch (...) {
@@ -707,8 +759,7 @@ ncsv_status ncsv_frames_sample_packed(ncsv_frames *f, uint64_t shots, uint64_t s
         const uint64_t W = (shots + 63) / 64;
         if (f->m && W && !out) fail(NCSV_INVALID, "null output");
         uint64_t *rec = sample_device(f, shots, seed);
-        if (f->m && W)
-            check_hip(hipMemcpyAsync(out, rec, f->m * W * 8, hipMemcpyDeviceToHost, f->stream), "download records");
+        if (f->m && W) download(f, out, rec, f->m * W * 8);
         check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
     });
 }

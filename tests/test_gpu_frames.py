@@ -38,6 +38,21 @@ def test_detectors_equal_the_oracle():
         assert all(np.array_equal(a, b) for a, b in zip(gd + go, wd + wo))
 
 
+def test_large_downloads_through_pinned_staging(monkeypatch):
+    """Records and detectors of 144-162 MiB come back through the pinned
+    staging buffers bit-identical to one plain copy of each."""
+    text = repetition_code(9, 9, 0.01)
+    hs = H.HipFrameSampler(H.Circuit.parse(text), 3)
+    shots = (1 << 24) - 5
+    got = [hs.sample_packed(shots, 11), hs.sample_detectors(shots, 12)]
+    monkeypatch.setenv("NCSV_FRAMES_STAGE", "0")
+    want = [hs.sample_packed(shots, 11), hs.sample_detectors(shots, 12)]
+    assert all(np.array_equal(a, b) for a, b in zip(got[0], want[0]))
+    for g, w in zip(got[1], want[1]):
+        assert all(np.array_equal(a, b) for a, b in zip(g, w))
+    assert sum(len(a) for a in got[0]) * 8 >= 128 << 20
+
+
 def popcount(words):
     return int(sum(bin(int(w)).count("1") for w in words))
 
