@@ -96,6 +96,10 @@ void free_state(ncsv_state *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->amp) (void)hipFree(s->amp);
     if (s->spare) (void)hipFree(s->spare);
+    for (int k = 0; k < 2; k++) {
+        if (s->stage[k]) (void)hipHostFree(s->stage[k]);
+        if (s->staged[k]) (void)hipEventDestroy(s->staged[k]);
+    }
     if (s->d_part) (void)hipFree(s->d_part);
     if (s->d_tab) (void)hipFree(s->d_tab);
     if (s->d_samp) (void)hipFree(s->d_samp);

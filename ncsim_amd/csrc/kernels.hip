@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "jit.hpp"
@@ -679,18 +680,91 @@ void fill_const(ncsv_state *s, double re) {
     }
 }
 
+// Copies between the device and the caller's (pageable) memory, done when
+// they return.  From 128 MiB on they go through two pinned 32 MiB staging
+// buffers: host threads fill or drain one while the other crosses the link
+// (a pageable buffer is otherwise copied a page-locked piece at a time).
+// NCSV_STAGE=0 keeps one plain copy.
+static bool use_staging(const ncsv_state *s, size_t bytes) {
+    constexpr size_t CH = size_t(32) << 20;
+    const char *e = getenv("NCSV_STAGE");
+    if (bytes < 4 * CH || (e && e[0] == '0')) return false;
+    for (int k = 0; k < 2; k++) {
+        auto *m = const_cast<ncsv_state *>(s);
+        if (!m->stage[k]) check_hip(hipHostMalloc(&m->stage[k], CH, hipHostMallocDefault), "hipHostMalloc(staging)");
+        if (!m->staged[k]) check_hip(hipEventCreateWithFlags(&m->staged[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    return true;
+}
+
+static void host_copy(void *dst, const void *src, size_t len) {
+    const unsigned T = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const size_t part = (len + T - 1) / T;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) {
+        const size_t a = std::min(len, t * part), b = std::min(len, a + part);
+        if (a < b) th.emplace_back([=]() { memcpy((char *)dst + a, (const char *)src + a, b - a); });
+    }
+    for (auto &x : th) x.join();
+}
+
+static void to_host(const ncsv_state *s, void *dst, const void *src, size_t bytes) {
+    constexpr size_t CH = size_t(32) << 20;
+    if (!use_staging(s, bytes)) {
+        check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream), "download");
+        sync(s);
+        return;
+    }
+    const size_t nch = (bytes + CH - 1) / CH;
+    auto drain = [&](size_t c) {
+        const size_t off = c * CH;
+        host_copy((char *)dst + off, s->stage[c & 1], std::min(CH, bytes - off));
+    };
+    for (size_t c = 0; c < nch; c++) {
+        const size_t off = c * CH;
+        check_hip(hipMemcpyAsync(s->stage[c & 1], (const char *)src + off, std::min(CH, bytes - off),
+                                 hipMemcpyDeviceToHost, s->stream),
+                  "download");
+        check_hip(hipEventRecord(s->staged[c & 1], s->stream), "hipEventRecord");
+        if (c > 0) {
+            check_hip(hipEventSynchronize(s->staged[(c - 1) & 1]), "hipEventSynchronize");
+            drain(c - 1);
+        }
+    }
+    check_hip(hipEventSynchronize(s->staged[(nch - 1) & 1]), "hipEventSynchronize");
+    drain(nch - 1);
+}
+
+static void to_device(const ncsv_state *s, void *dst, const void *src, size_t bytes) {
+    constexpr size_t CH = size_t(32) << 20;
+    if (!use_staging(s, bytes)) {
+        check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), "upload");
+        sync(s);
+        return;
+    }
+    const size_t nch = (bytes + CH - 1) / CH;
+    for (size_t c = 0; c < nch; c++) {
+        const size_t off = c * CH, len = std::min(CH, bytes - off);
+        // the copy that last read this staging buffer (two chunks back) must be done
+        if (c >= 2) check_hip(hipEventSynchronize(s->staged[c & 1]), "hipEventSynchronize");
+        host_copy(s->stage[c & 1], (const char *)src + off, len);
+        check_hip(hipMemcpyAsync((char *)dst + off, s->stage[c & 1], len, hipMemcpyHostToDevice, s->stream), "upload");
+        check_hip(hipEventRecord(s->staged[c & 1], s->stream), "hipEventRecord");
+    }
+    sync(s);
+}
+
 void upload(ncsv_state *s, const double *amp) {
     const uint64_t N = 1ull << s->n;
     s->lscale = 1.0;
     if (s->prec == NCSV_C128) {
-        check_hip(hipMemcpyAsync(s->amp, amp, s->amp_bytes(), hipMemcpyHostToDevice, s->stream), "upload");
-        sync(s);
+        to_device(s, s->amp, amp, s->amp_bytes());
         return;
     }
     // complex64: stage FP64 on the device, convert
     double *tmp = nullptr;
     check_hip(hipMalloc(&tmp, N * 16), "hipMalloc(staging)");
-    check_hip(hipMemcpyAsync(tmp, amp, N * 16, hipMemcpyHostToDevice, s->stream), "upload");
+    to_device(s, tmp, amp, N * 16);
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(2 * N)), dim3(TPB), 0, s->stream, tmp,
                        (float *)s->amp, 2 * N, 1.0);
     launch_check("k_convert");
@@ -701,8 +775,7 @@ void upload(ncsv_state *s, const double *amp) {
 void download(const ncsv_state *s, double *out) {
     const uint64_t N = 1ull << s->n;
     if (s->prec == NCSV_C128) {
-        check_hip(hipMemcpyAsync(out, s->amp, s->amp_bytes(), hipMemcpyDeviceToHost, s->stream), "download");
-        sync(s);
+        to_host(s, out, s->amp, s->amp_bytes());
         return;
     }
     double *tmp = nullptr;
@@ -710,8 +783,7 @@ void download(const ncsv_state *s, double *out) {
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(2 * N)), dim3(TPB), 0, s->stream,
                        (const float *)s->amp, tmp, 2 * N, s->lscale);
     launch_check("k_convert");
-    check_hip(hipMemcpyAsync(out, tmp, N * 16, hipMemcpyDeviceToHost, s->stream), "download");
-    sync(s);
+    to_host(s, out, tmp, N * 16);
     (void)hipFree(tmp);
 }
 

@@ -132,6 +132,10 @@ struct ncsv_state {
     // spare_failed: the allocation was refused, sweeps stay in place
     void *spare = nullptr;
     bool spare_failed = false;
+    // pinned staging for large copies between the device and the caller's
+    // memory (allocated on the first such copy)
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t staged[2] = {nullptr, nullptr};
     double lscale = 1.0;  // logical amplitude = lscale * stored (complex64 form)
     // set_basis deferred: the state is |pending_basis> but the amplitudes are
     // not written yet (-1: they are); a run's first specialized sweep computes

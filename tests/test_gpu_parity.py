@@ -291,6 +291,25 @@ def run_both(text, seed, fusion=None):
 
 
 @pytest.mark.parametrize("precision", [H.C128, H.C64])
+def test_large_copies_through_pinned_staging(precision, monkeypatch):
+    """set_amplitudes and to_vector of 256 MiB go through the pinned staging
+    buffers (chunked, threaded host copies) and equal one plain copy each."""
+    n = 24
+    rng = np.random.default_rng(3)
+    a = (rng.random(1 << n) - 0.5) + 1j * (rng.random(1 << n) - 0.5)
+    got = []
+    for stage in ("1", "0"):
+        monkeypatch.setenv("NCSV_STAGE", stage)
+        d = H.HipStateVector.from_amplitudes(n, a, precision)
+        got.append(d.to_vector())
+    assert np.array_equal(got[0], got[1])
+    if precision == H.C128:
+        assert np.array_equal(got[0], a)
+    else:
+        assert np.abs(got[0] - a).max() < 1e-6
+
+
+@pytest.mark.parametrize("precision", [H.C128, H.C64])
 def test_out_of_place_sweeps_equal_in_place(precision, monkeypatch):
     """NCSV_JIT_OOP=1: the specialized sweeps write a second buffer and the
     two swap after each sweep; states, expectations and a second run on the
