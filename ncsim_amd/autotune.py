@@ -85,7 +85,7 @@ def choose(timings):
     return max(near, key=lambda t: (t[0]["tiles_per_item"], -t[1]))[0]
 
 
-ITEMS = (32, 64, 128)  # queue item sizes timed on the whole batch
+ITEMS = (32, 64, 128, 256)  # queue item sizes timed on the whole batch
 
 
 def tune_batch(batch, sample=96, reps=2, cache=None, force=False, log=None, full_budget_s=2.0):
