@@ -651,7 +651,13 @@ void download(ncsv_frames *f, void *dst, const void *src, size_t bytes) {
         return;
     }
     for (int k = 0; k < 2; k++) {
-        if (!f->stage[k]) check_hip(hipHostMalloc(&f->stage[k], CH, hipHostMallocDefault), "hipHostMalloc(staging)");
+        if (!f->stage[k] && hipHostMalloc(&f->stage[k], CH, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();  // no pinned memory to be had: one plain copy
+            f->stage[k] = nullptr;
+            check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, f->stream), "download");
+            check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
+            return;
+        }
         if (!f->staged[k]) check_hip(hipEventCreateWithFlags(&f->staged[k], hipEventDisableTiming), "hipEventCreate");
     }
     const unsigned T = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));

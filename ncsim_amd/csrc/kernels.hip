@@ -689,9 +689,13 @@ static bool use_staging(const ncsv_state *s, size_t bytes) {
     constexpr size_t CH = size_t(32) << 20;
     const char *e = getenv("NCSV_STAGE");
     if (bytes < 4 * CH || (e && e[0] == '0')) return false;
+    auto *m = const_cast<ncsv_state *>(s);
     for (int k = 0; k < 2; k++) {
-        auto *m = const_cast<ncsv_state *>(s);
-        if (!m->stage[k]) check_hip(hipHostMalloc(&m->stage[k], CH, hipHostMallocDefault), "hipHostMalloc(staging)");
+        if (!m->stage[k] && hipHostMalloc(&m->stage[k], CH, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();  // no pinned memory to be had: one plain copy
+            m->stage[k] = nullptr;
+            return false;
+        }
         if (!m->staged[k]) check_hip(hipEventCreateWithFlags(&m->staged[k], hipEventDisableTiming), "hipEventCreate");
     }
     return true;
