@@ -418,6 +418,9 @@ struct ncsv_frames {
     // pinned staging for large downloads into the caller's memory
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t staged[2] = {nullptr, nullptr};
+    // the parity kernel's tables and output, kept between calls (grown on demand)
+    void *d_par = nullptr;
+    size_t par_cap = 0;
     ~ncsv_frames() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (d_ops) (void)hipFree(d_ops);
@@ -427,6 +430,7 @@ struct ncsv_frames {
             if (stage[k]) (void)hipHostFree(stage[k]);
             if (staged[k]) (void)hipEventDestroy(staged[k]);
         }
+        if (d_par) (void)hipFree(d_par);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -694,37 +698,33 @@ void parities_device(ncsv_frames *f, const uint64_t *rec, uint64_t shots, const 
     std::vector<uint32_t> off, idx;
     std::vector<uint8_t> flip;
     parity_lists(f, sets, &off, &idx, &flip);
-    uint32_t *d_off = nullptr, *d_idx = nullptr;
-    uint8_t *d_flip = nullptr;
-    uint64_t *d_out = nullptr;
-    auto cleanup = [&]() {
-        if (d_off) (void)hipFree(d_off);
-        if (d_idx) (void)hipFree(d_idx);
-        if (d_flip) (void)hipFree(d_flip);
-        if (d_out) (void)hipFree(d_out);
-    };
-    try {
-        check_hip(hipMalloc(&d_off, off.size() * 4), "hipMalloc");
-        check_hip(hipMalloc(&d_idx, std::max<size_t>(1, idx.size()) * 4), "hipMalloc");
-        check_hip(hipMalloc(&d_flip, flip.size()), "hipMalloc");
-        check_hip(hipMalloc(&d_out, std::max<uint64_t>(1, sets.size() * W) * 8), "hipMalloc");
-        check_hip(hipMemcpyAsync(d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, f->stream), "upload");
-        if (!idx.empty())
-            check_hip(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, f->stream), "upload");
-        check_hip(hipMemcpyAsync(d_flip, flip.data(), flip.size(), hipMemcpyHostToDevice, f->stream), "upload");
-        const uint64_t items = sets.size() * W;
-        if (items) {
-            hipLaunchKernelGGL(k_parities, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, f->stream, rec, W, shots,
-                               d_off, d_idx, d_flip, (uint32_t)sets.size(), d_out);
-            check_hip(hipGetLastError(), "k_parities");
-            download(f, host_out, d_out, items * 8);
-        }
-        check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
-    } catch (...) {
-        cleanup();
-        throw;
+    // one device buffer for the output (8-byte words first), the offsets, the
+    // indices and the flips, kept on the sampler between calls
+    const uint64_t items = sets.size() * W;
+    auto up256 = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t o_off = up256(std::max<uint64_t>(1, items) * 8), o_idx = o_off + up256(off.size() * 4),
+                 o_flip = o_idx + up256(std::max<size_t>(1, idx.size()) * 4), total = o_flip + up256(flip.size() + 1);
+    if (total > f->par_cap) {
+        if (f->d_par) check_hip(hipFree(f->d_par), "hipFree");
+        f->d_par = nullptr;
+        f->par_cap = 0;
+        check_hip(hipMalloc(&f->d_par, total), "hipMalloc(parities)");
+        f->par_cap = total;
     }
-    cleanup();
+    uint64_t *d_out = (uint64_t *)f->d_par;
+    uint32_t *d_off = (uint32_t *)((char *)f->d_par + o_off), *d_idx = (uint32_t *)((char *)f->d_par + o_idx);
+    uint8_t *d_flip = (uint8_t *)((char *)f->d_par + o_flip);
+    check_hip(hipMemcpyAsync(d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, f->stream), "upload");
+    if (!idx.empty())
+        check_hip(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, f->stream), "upload");
+    check_hip(hipMemcpyAsync(d_flip, flip.data(), flip.size(), hipMemcpyHostToDevice, f->stream), "upload");
+    if (items) {
+        hipLaunchKernelGGL(k_parities, dim3((uint32_t)((items + 255) / 256)), dim3(256), 0, f->stream, rec, W, shots,
+                           d_off, d_idx, d_flip, (uint32_t)sets.size(), d_out);
+        check_hip(hipGetLastError(), "k_parities");
+        download(f, host_out, d_out, items * 8);
+    }
+    check_hip(hipStreamSynchronize(f->stream), "hipStreamSynchronize");
 }
 
 }  // namespace
