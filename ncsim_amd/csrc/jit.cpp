@@ -1560,6 +1560,10 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
         emit_tile("bi", "0", "a", "    ");
         o << "  }\n";
     }
+    // which of each Pauli's sums (bit 0 real, bit 1 imaginary) receives any
+    // term: the other stays out of the loop (adding its signed zeros would
+    // keep a register and an add per Pauli live)
+    std::vector<uint8_t> used(cnt, 0);
     uint32_t cur = memr;
     for (auto &st : eg.stages) {
         if (st.rpos != cur) {
@@ -1592,6 +1596,7 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
             auto add = [&](int kk, const std::string &re, const std::string &im) {
                 // (tr, ti) += i^kk (re + i im); an empty im is zero
                 const bool hi = !im.empty();
+                used[k] |= hi ? 3 : (kk & 1) ? 2 : 1;
                 switch (kk & 3) {
                 case 0: o << "    tr += " << re << ";" << (hi ? " ti += " + im + ";" : "") << "\n"; break;
                 case 1: o << (hi ? "    tr -= " + im + ";" : "   ") << " ti += " << re << ";\n"; break;
@@ -1624,10 +1629,12 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
                     o << "    }\n";
                 }
             }
+            const std::string ar = (used[k] & 1) ? " sr" + std::to_string(k) + (xr ? " += f2*tr;" : " += tr;") : "",
+                              ai = (used[k] & 2) ? " si" + std::to_string(k) + (xr ? " += f2*ti;" : " += ti;") : "";
             if (xr)
-                o << "    const double f2 = odd ? -2.0 : 2.0;\n    sr" << k << " += f2*tr; si" << k << " += f2*ti; }\n";
+                o << "    const double f2 = odd ? -2.0 : 2.0;\n   " << ar << ai << " }\n";
             else
-                o << "    if (odd) { tr = -tr; ti = -ti; }\n    sr" << k << " += tr; si" << k << " += ti; }\n";
+                o << "    if (odd) { tr = -tr; ti = -ti; }\n   " << ar << ai << " }\n";
             // one Pauli at a time: interleaving them raised the registers past two wavefronts per SIMD
             o << "    __builtin_amdgcn_sched_barrier(0);\n";
         }
@@ -1642,8 +1649,10 @@ std::string generate_expect(const ExpGroup &eg, uint32_t n, const TileGeom &geo,
     o << "  }\n";
     o << "  const int lane = tid & 63, wv = tid >> 6;\n";
     for (uint32_t k = 0; k < cnt; k++) {
-        o << "  { const double a = wsum(sr" << k << "), b2 = wsum(si" << k << "); if (lane == 0) { red[wv][" << 2 * k
-          << "] = a; red[wv][" << 2 * k + 1 << "] = b2; } }\n";
+        const std::string a = (used[k] & 1) ? "wsum(sr" + std::to_string(k) + ")" : "0.0",
+                          b2 = (used[k] & 2) ? "wsum(si" + std::to_string(k) + ")" : "0.0";
+        o << "  { const double a = " << a << ", b2 = " << b2 << "; if (lane == 0) { red[wv][" << 2 * k << "] = a; red[wv]["
+          << 2 * k + 1 << "] = b2; } }\n";
     }
     o << "  __syncthreads();\n  if (tid < " << 2 * cnt << ") { double s = 0.0; for (int i = 0; i < " << ((1u << g.LT) / 64)
       << "; i++) s += red[i][tid]; part[(unsigned long long)tid * gridDim.x + blockIdx.x] = s; }\n}\n";
@@ -2056,7 +2065,11 @@ bool expectations(ncsv_state *s, uint32_t count, const uint64_t *x, const uint64
     std::vector<void *> kern;
     for (auto &eg : groups) kern.push_back(prepare_named(generate_expect(eg, s->n, geo, dbl), "ncsv_expect"));
     const uint64_t tiles = 1ull << (s->n - L);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, 2048);
+    // persistent grid (a multiple of 8 keeps each XCD on its own run of
+    // tiles); NCSV_EXPECT_GRID sets it
+    uint64_t want = 2048;
+    if (const char *e = getenv("NCSV_EXPECT_GRID")) want = std::max<uint64_t>(8, strtoull(e, nullptr, 10) & ~7ull);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, want);
     dev::ensure_part(s, (size_t)2 * GMAX * grid + 2 * GMAX + 64);
     std::vector<double> res(2 * GMAX);
     try {
