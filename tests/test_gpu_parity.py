@@ -633,6 +633,27 @@ def test_complex64_multi_pauli_expectations(fusion):
     assert max(abs(o.expectation(p) - g) for p, g in zip(ps, got)) < 2e-5
 
 
+@pytest.mark.parametrize("precision,tol", [(H.C128, 1e-12), (H.C64, 2e-5)])
+def test_multi_pauli_expectations_with_any_phase(precision, tol):
+    """Paulis whose phase is not the Hermitian one (i^k P with k shifted by
+    one or three) have purely imaginary expectations: the grouped kernels at
+    22 qubits put those terms on the imaginary sums, Hermitian ones on the
+    real sums, and both match the oracle (mod.rs:102-108)."""
+    n = 22
+    text = W.rotation_brickwork(n, 10, 21)
+    o = StateVector.zero_state(n)
+    o.run(OCircuit.parse(text), SplitMix64(0))
+    d = H.HipStateVector.zero_state(n, precision)
+    d.run(H.Circuit.parse(text))
+    r = W.SplitMix64(22)
+    ps = [Pauli(n, x, z, (ph + r.below(4)) & 3) for x, z, ph in W.random_paulis(n, 40, 23)]
+    want = [o.expectation(p) for p in ps]
+    got = d.expectations(ps)
+    assert max(abs(w - g) for w, g in zip(want, got)) < tol
+    # both sums carry values in this set
+    assert max(abs(w.imag) for w in want) > 1e-3 and max(abs(w.real) for w in want) > 1e-3
+
+
 def test_complex64_thirteen_pauli_group_on_a_small_state(monkeypatch):
     """NCSV_EXPECT_GMAX=13 under fusion 2 on 16 qubits: a 13-Pauli group on a
     complex64 state smaller than the default specialization size."""
