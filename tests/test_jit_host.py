@@ -6,9 +6,11 @@ and compared with the plan's FP64 host emulation (ncsv_plan_emulate).  This
 exercises the generator's register renaming, every LDS exchange form
 (half-tile passes on a shared register position, wavefront turns,
 intermediate layouts, whole-tile), the lazily applied diagonal factors and
-their thread and block-index tables, without a GPU."""
+their thread and block-index tables, without a GPU.  The grouped expectation
+kernels run the same way, with __shfl_down emulated per wavefront."""
 import ctypes
 import os
+import re
 import shutil
 import subprocess
 import tempfile
@@ -413,5 +415,93 @@ def test_from_basis_sweep_on_host_threads(precision, index, monkeypatch):
         got = buf.astype(np.complex128)
         assert not np.isnan(got).any()
         assert np.abs(got - want).max() < (1e-13 if precision == H.C128 else 2e-6)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+EXPECT_PRE = r'''
+struct GD { unsigned x; };
+static GD gridDim;
+// __shfl_down over a wavefront of host threads: every thread of the
+// workgroup makes the same calls in the same order (wsum after the tile loop)
+static double g_sd[4][64];
+inline double __shfl_down(double v, int k, int w) {
+    const unsigned wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    g_sd[wv][l] = v;
+    g_bar->arrive_and_wait();
+    const double r = (int)l + k < w ? g_sd[wv][l + k] : v;
+    g_bar->arrive_and_wait();
+    return r;
+}
+inline void lds_bar() { g_bar->arrive_and_wait(); }
+'''
+
+POST_EXPECT = r'''
+extern "C" void run_host_expect(const V *amp, double *part, unsigned long long ntiles, unsigned grid, unsigned nthreads) {
+    gridDim.x = grid;
+    for (unsigned b = 0; b < grid; b++) {
+        std::barrier<> bar(nthreads);
+        g_bar = &bar;
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nthreads; t++)
+            th.emplace_back([=] { threadIdx.x = t; blockIdx.x = b; ncsv_expect(amp, part, ntiles); });
+        for (auto &x : th) x.join();
+    }
+}
+'''
+
+
+@pytest.mark.parametrize("precision,prefetch", [(H.C128, "1"), (H.C128, "0"), (H.C64, "0")])
+def test_expectation_kernels_on_host_threads(precision, prefetch, monkeypatch):
+    """The grouped expectation kernels the generator writes for 40 Paulis on
+    16 qubits (phases shifted off the Hermitian ones, so both the real and
+    the imaginary sums carry values), run on host threads: each group's
+    per-Pauli sums equal the oracle's expectations (mod.rs:102-108), each
+    matched to a distinct Pauli."""
+    from oracle.pauli import Pauli
+    from oracle.statevector import StateVector
+
+    n = 16
+    work = tempfile.mkdtemp()
+    try:
+        monkeypatch.setenv("NCSV_JIT_DUMP", work)
+        monkeypatch.setenv("NCSV_JIT_NT", "0")
+        monkeypatch.setenv("NCSV_EXPECT_PREFETCH", prefetch)
+        r = W.SplitMix64(41)
+        ps = [(x, z, (ph + r.below(4)) & 3) for x, z, ph in W.random_paulis(n, 40, 42)]
+        k = len(ps)
+        xs = (ctypes.c_uint64 * k)(*[p[0] for p in ps])
+        zs = (ctypes.c_uint64 * k)(*[p[1] for p in ps])
+        ph = (ctypes.c_uint32 * k)(*[p[2] for p in ps])
+        g = ctypes.c_uint32()
+        assert L.lib().ncsv_jit_check_expect(n, int(precision), k, xs, zs, ph, b"gfx950", ctypes.byref(g)) == 0
+        srcs = [open(os.path.join(work, f)).read() for f in sorted(os.listdir(work)) if f.endswith(".hip")]
+        srcs = [s for s in srcs if "ncsv_expect" in s]
+        assert len(srcs) == g.value >= 2
+        rng = np.random.default_rng(7)
+        a = (rng.random(1 << n) - 0.5) + 1j * (rng.random(1 << n) - 0.5)
+        a /= np.linalg.norm(a)
+        buf = a.copy() if precision == H.C128 else a.astype(np.complex64)
+        o = StateVector.from_amplitudes(n, buf.astype(np.complex128))
+        want = [o.expectation(Pauli(n, x, z, p)) for x, z, p in ps]
+        assert max(abs(w.imag) for w in want) > 1e-3 and max(abs(w.real) for w in want) > 1e-3
+        tile = 12 if precision == H.C128 else 13
+        ntiles = 1 << (n - tile)
+        got = []
+        for i, src in enumerate(srcs):
+            two_cnt = int(re.search(r"red\[\d+\]\[(\d+)\]", src).group(1))
+            lib = host_kernel(EXPECT_PRE + src, work, f"e{i}", POST_EXPECT)
+            part = np.zeros(two_cnt * ntiles)
+            lib.run_host_expect(buf.ctypes.data_as(ctypes.c_void_p), part.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.c_ulonglong(ntiles), ctypes.c_uint(ntiles), ctypes.c_uint(256))
+            sums = part.reshape(two_cnt, ntiles).sum(axis=1)
+            got += [complex(sums[2 * j], sums[2 * j + 1]) for j in range(two_cnt // 2)]
+        assert len(got) == len(want)
+        tol = 1e-13 if precision == H.C128 else 1e-6
+        free = list(range(len(want)))
+        for v in got:
+            j = min(free, key=lambda j: abs(want[j] - v))
+            assert abs(want[j] - v) < tol, (v, want[j])
+            free.remove(j)
     finally:
         shutil.rmtree(work, ignore_errors=True)
